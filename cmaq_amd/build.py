@@ -1,0 +1,103 @@
+"""Build libcmaq_ros3.so: generate the per-mechanism kernels and compile
+everything for gfx950 with hipcc, in-tree.
+
+    python -m cmaq_amd.build            # all mechanisms under cmaq_amd/mechs
+"""
+
+import concurrent.futures as cf
+import json
+import os
+import subprocess
+import sys
+
+from .codegen.emit_hip import emit
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+GEN = os.path.join(CSRC, "gen")
+OBJ = os.path.join(GEN, "obj")
+LIB = os.path.join(HERE, "libcmaq_ros3.so")
+MECHS = {"cb6r5_ae7_aq": "cb6r5_ae7_aq.json", "cracmm2": "cracmm2.json"}
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# -ffp-contract=off: the reference rounds every multiply and add separately.
+# packed FP32 is disabled for all device code on this platform.
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+             "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+
+def load_mech(name):
+    with open(os.path.join(HERE, "mechs", MECHS[name])) as f:
+        return json.load(f)
+
+
+def _write_if_changed(path, text):
+    if os.path.exists(path) and open(path).read() == text:
+        return False
+    with open(path, "w") as f:
+        f.write(text)
+    return True
+
+
+def generate(names=None):
+    os.makedirs(GEN, exist_ok=True)
+    names = names or list(MECHS)
+    srcs = []
+    for name in names:
+        text, _ = emit(load_mech(name), name)
+        path = os.path.join(GEN, f"ros3_{name}.hip")
+        _write_if_changed(path, text)
+        srcs.append(path)
+    reg = ["// Generated by cmaq_amd/build.py; do not edit.", "#include \"../ros3_registry.h\""]
+    for name in names:
+        reg.append(f"extern \"C\" const ros3::MechKernels ros3_kernels_{name};")
+    reg.append("extern \"C\" const ros3::MechKernels *const ros3_mech_registry[] = {")
+    reg += [f"    &ros3_kernels_{name}," for name in names]
+    reg.append("};")
+    reg.append(f"extern \"C\" const int ros3_mech_registry_size = {len(names)};")
+    path = os.path.join(GEN, "registry.cpp")
+    _write_if_changed(path, "\n".join(reg) + "\n")
+    srcs.append(path)
+    return srcs
+
+
+def _compile(src, obj, host_only=False):
+    deps = [src, os.path.join(CSRC, "ros3_device.h"), os.path.join(CSRC, "ros3_solver_kernel.inc"),
+            os.path.join(CSRC, "ros3_registry.h"), os.path.join(HERE, "..", "include", "cmaq_ros3.h")]
+    if os.path.exists(obj) and all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps if os.path.exists(d)):
+        return obj
+    if host_only:
+        cmd = ["g++", "-O2", "-fPIC", "-std=c++17", "-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(names=None, verbose=True):
+    os.makedirs(OBJ, exist_ok=True)
+    gen_srcs = generate(names)
+    jobs = []
+    for src in gen_srcs:
+        base = os.path.splitext(os.path.basename(src))[0]
+        jobs.append((src, os.path.join(OBJ, base + ".o"), src.endswith("registry.cpp")))
+    jobs.append((os.path.join(CSRC, "cmaq_ros3.cpp"), os.path.join(OBJ, "cmaq_ros3.o"), False))
+    workers = int(os.environ.get("MAX_JOBS", "8"))
+    with cf.ThreadPoolExecutor(max_workers=max(1, min(workers, len(jobs)))) as ex:
+        objs = list(ex.map(lambda j: _compile(*j), jobs))
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(LIB + ".tmp", LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build(sys.argv[1:] or None)

@@ -1,0 +1,552 @@
+// Host runtime of the gfx950 ROS3 solver: the C ABI declared in
+// include/cmaq_ros3.h.
+//
+// cmaq_ros3_init  ~ RBINIT + RBSPARSE (rbinit.F:26, rbsparse.F:26): checks the
+//                   RXNS_DATA tables, decodes the rate-constant program and
+//                   selects the generated kernels built for this mechanism.
+// cmaq_ros3_run*  ~ CALC_RCONST + RBSOLVER over all cells
+//                   (rbdriver.F:440-677): one rates kernel, then the
+//                   persistent queue-refilled solver kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cmaq_ros3.h"
+#include "ros3_device.h"
+
+using namespace ros3;
+
+namespace {
+
+constexpr int kMaxRates = 640;
+constexpr int kMaxMul = 64;
+constexpr int kSolveBlock = 256;
+
+thread_local std::string g_last_error = "";
+
+}  // namespace
+
+struct cmaq_ros3_ctx {
+    int device = 0;
+    const MechKernels *mk = nullptr;
+    int ns = 0, nr = 0, nphotab = 0, nhetero = 0;
+    std::vector<RateOp> ops;
+    std::vector<RateMul> mul;
+    std::vector<double> rtol_new, atol_new;
+    int32_t max_attempts = 1000000;
+    int32_t refill_min = 0;
+    int waves_per_cu = 0;
+    int grid_blocks = 0;      // solver workgroups
+    int64_t waves = 0;
+    // device buffers
+    RateOp *d_ops = nullptr;
+    RateMul *d_mul = nullptr;
+    double *d_rtol = nullptr, *d_atol = nullptr;
+    double *d_ws = nullptr;
+    unsigned long long *d_queue = nullptr;   // [0] queue, then fail record
+    int32_t *d_fail_flag = nullptr, *d_fail_code = nullptr;
+    int64_t *d_fail_cell = nullptr;
+    double *d_fail_info = nullptr;
+    void *d_small = nullptr;
+    double *d_rki = nullptr;
+    int64_t rki_cells = 0;
+    // staging for host-pointer runs
+    void *d_io = nullptr;
+    size_t io_bytes = 0;
+    std::string err;
+};
+
+static int set_err(cmaq_ros3_ctx *c, int code, const std::string &msg)
+{
+    g_last_error = msg;
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                         \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess)                                                      \
+            return set_err(ctx, CMAQ_ROS3_E_HIP,                                   \
+                           std::string(#expr) + ": " + hipGetErrorString(_e));     \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// table access (Fortran column-major arrays, 1-based ids inside)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Tab {
+    const cmaq_ros3_mech *m;
+    int nr() const { return m->nrxns; }
+    int irr(int r, int s) const { return m->irr[(size_t)s * m->nrxns + r]; }
+    double sc(int r, int p) const { return m->sc[(size_t)p * m->nrxns + r]; }
+    double rtdat(int k, int r) const { return m->rtdat[(size_t)r * 3 + k]; }
+    double rfdat(int k, int f) const { return m->rfdat[(size_t)f * 5 + k]; }
+    int iph(int n, int k) const { return m->iph[(size_t)k * m->nmphot + n]; }
+    int ihet(int n, int k) const { return m->ihetero[(size_t)k * m->mhetero + n]; }
+};
+
+struct Fnv {
+    unsigned long long h = 0xcbf29ce484222325ull;
+    void bytes(const void *p, size_t n)
+    {
+        const unsigned char *b = (const unsigned char *)p;
+        for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 0x100000001b3ull; }
+    }
+    void i32(int32_t v) { bytes(&v, 4); }
+    void f64(double v) { bytes(&v, 8); }
+};
+
+// Same serialisation as cmaq_amd/codegen/emit_hip.py: structure_hash.
+unsigned long long structure_hash(const Tab &t)
+{
+    Fnv f;
+    f.i32(t.m->numb_mech_spc); f.i32(t.m->nrxns); f.i32(t.m->mxprd);
+    for (int r = 0; r < t.nr(); ++r) {
+        f.i32(t.m->nreact[r]); f.i32(t.m->nprdct[r]); f.i32(t.m->irxbits[r]);
+        for (int k = 0; k < t.m->nreact[r]; ++k) f.i32(t.irr(r, k));
+        for (int k = 0; k < t.m->nprdct[r]; ++k) { f.i32(t.irr(r, 3 + k)); f.f64(t.sc(r, k)); }
+    }
+    return f.h;
+}
+
+// chemmech prints derived constants with FORMAT 1PD12.4
+double d12_4(double v)
+{
+    char b[64];
+    std::snprintf(b, sizeof b, "%.4e", v);
+    return std::strtod(b, nullptr);
+}
+
+// WRITE_RATE_CONVERT_BEFORE/AFTER (BASIC_WRITE_ROUTINES.f:1102-1152)
+int conv_of(int order)
+{
+    switch (order) {
+    case 0: return CV_INV_CFACT;
+    case 1: return CV_SFACT;
+    case 2: return CV_CFACT;
+    case 3: return CV_CFACT_SQU;
+    case -1: return CV_RFACT;
+    case -2: return CV_RFACT_SQU;
+    default: return CV_NONE;
+    }
+}
+
+// Rate program from the tables, following the expressions chemmech writes
+// into CALC_RCONST (wrt_reactions_module.f:580-800).
+int decode_rates(const Tab &t, std::vector<RateOp> &ops, std::vector<RateMul> &mul, std::string &why)
+{
+    const cmaq_ros3_mech *m = t.m;
+    const int nr = m->nrxns;
+    ops.assign(nr, RateOp{});
+    auto fall = [&](int r) {
+        for (int f = 0; f < m->nfalloff; ++f)
+            if (m->irrfall[f] == r + 1) return f;
+        return -1;
+    };
+    std::vector<int> phot(nr, -1);
+    for (int n = 0; n < m->nmphot; ++n) {
+        int r = t.iph(n, 0) - 1;
+        if (r < 0) continue;
+        if (r >= nr || m->ktype[r] != 0) { why = "IPH row " + std::to_string(n + 1) + " is not a photolysis reaction"; return -1; }
+        if (t.iph(n, 2) == 0) { why = "photolysis relative to another reaction (IPH(:,3)=0) not supported"; return -1; }
+        int tab = t.iph(n, 1) - 1;
+        if (tab < 0 || tab >= m->nphotab) { why = "IPH table index out of range"; return -1; }
+        if (phot[r] < 0) phot[r] = tab;
+    }
+    for (int r = 0; r < nr; ++r) {
+        RateOp &op = ops[r];
+        op.kind = RK_NONE; op.conv = CV_NONE; op.ref = 0; op.scaled = 0;
+        for (double &x : op.p) x = 0.0;
+        const int kt = m->ktype[r], ord = m->iorder[r];
+        const double a = t.rtdat(0, r), b = t.rtdat(1, r), c = t.rtdat(2, r);
+        const int f = fall(r);
+        const int conv_kunits = (m->kunits == 2) ? conv_of(ord) : CV_NONE;
+        switch (kt) {
+        case 0:
+            if (phot[r] >= 0) {
+                op.kind = RK_PHOT; op.ref = phot[r]; op.scaled = (a != 1.0); op.p[0] = a;
+            }
+            break;
+        case 12:
+            if (f < 0 || ord != 1) { why = "halogen rate " + std::to_string(r + 1) + " unsupported"; return -1; }
+            op.kind = RK_HALOGEN;
+            op.p[0] = a; op.p[1] = t.rfdat(0, f); op.p[2] = b; op.p[3] = t.rfdat(1, f); op.p[4] = c;
+            break;
+        case -1: {
+            int tab = -1;
+            for (int n = 0; n < m->mhetero; ++n)
+                if (t.ihet(n, 0) == r + 1) { tab = t.ihet(n, 1) - 1; break; }
+            if (tab < 0 || tab >= m->nhetero) { why = "no IHETERO row for reaction " + std::to_string(r + 1); return -1; }
+            op.kind = RK_HET; op.ref = tab; op.scaled = (a != 1.0); op.p[0] = a;
+            break;
+        }
+        case 1:
+            op.kind = RK_CONST; op.conv = conv_of(ord); op.p[0] = a;
+            break;
+        case 2:
+            op.kind = RK_T02; op.conv = conv_kunits; op.p[0] = a; op.p[1] = b;
+            break;
+        case 3:
+            op.kind = RK_T03; op.conv = conv_kunits; op.p[0] = a; op.p[1] = c;
+            break;
+        case 4:
+            op.kind = RK_T04; op.conv = conv_kunits; op.p[0] = a; op.p[1] = c; op.p[2] = b;
+            break;
+        case 5:
+        case 6: {
+            const int irx = (int)(kt == 5 ? c : b);   // 1-based reference reaction
+            if (irx < 1 || irx > nr) { why = "bad reference reaction for " + std::to_string(r + 1); return -1; }
+            if (irx > r + 1) break;                   // chemmech never writes these: RKI stays 0
+            const int idiff = ord - m->iorder[irx - 1];
+            const int kti = m->ktype[irx - 1];
+            const bool falloff = kti > 7 && kti < 11;
+            op.conv = (idiff != 0 && (m->kunits == 2 || falloff)) ? conv_of(idiff) : CV_NONE;
+            op.ref = irx - 1;
+            if (kt == 5) {
+                op.kind = RK_T05; op.p[0] = d12_4(1.0 / a); op.p[1] = -b;
+            } else {
+                op.kind = RK_T06; op.scaled = (a != 1.0); op.p[0] = a;
+            }
+            break;
+        }
+        case 8:
+            if (f < 0) { why = "no RFDAT row for " + std::to_string(r + 1); return -1; }
+            op.kind = RK_T08; op.conv = conv_of(ord);
+            op.p[0] = a; op.p[1] = b; op.p[2] = c;
+            op.p[3] = t.rfdat(0, f); op.p[4] = t.rfdat(1, f); op.p[5] = t.rfdat(2, f);
+            break;
+        case 9:
+            if (f < 0) { why = "no RFDAT row for " + std::to_string(r + 1); return -1; }
+            op.conv = conv_of(ord);
+            if (t.rfdat(1, f) == 0.0 && t.rfdat(2, f) == 0.0) {
+                op.kind = RK_T09; op.p[0] = a; op.p[1] = b; op.p[2] = c; op.p[3] = t.rfdat(0, f);
+            } else {
+                op.kind = RK_T11;
+                op.p[0] = a; op.p[1] = t.rfdat(1, f); op.p[2] = b; op.p[3] = c;
+                op.p[4] = t.rfdat(2, f); op.p[5] = t.rfdat(0, f); op.p[6] = t.rfdat(3, f); op.p[7] = t.rfdat(4, f);
+            }
+            break;
+        case 10:
+            if (f < 0) { why = "no RFDAT row for " + std::to_string(r + 1); return -1; }
+            op.kind = RK_T10; op.conv = conv_of(ord);
+            op.p[0] = a; op.p[1] = c; op.p[2] = b; op.p[3] = t.rfdat(0, f);
+            op.p[4] = t.rfdat(2, f); op.p[5] = t.rfdat(1, f); op.p[6] = t.rfdat(4, f); op.p[7] = t.rfdat(3, f);
+            break;
+        default:
+            why = "reaction " + std::to_string(r + 1) + ": rate type " + std::to_string(kt) + " not supported";
+            return -1;
+        }
+    }
+    mul.clear();
+    auto add = [&](int n, const int32_t *l, int h2o, double v) {
+        for (int q = 0; q < n; ++q) mul.push_back(RateMul{l[q] - 1, h2o, v});
+    };
+    add(m->nwm, m->nrxwm, 0, m->atm_air);
+    add(m->nwo2, m->nrxwo2, 0, m->atm_o2);
+    add(m->nwn2, m->nrxwn2, 0, m->atm_n2);
+    add(m->nww, m->nrxww, 1, 0.0);
+    add(m->nwh2, m->nrxwh2, 0, m->atm_h2);
+    add(m->nwch4, m->nrxwch4, 0, m->atm_ch4);
+    for (const RateMul &q : mul)
+        if (q.rxn < 0 || q.rxn >= nr) { why = "multiplier list references a bad reaction"; return -1; }
+    return 0;
+}
+
+int validate(const cmaq_ros3_mech *m, std::string &why)
+{
+    if (!m) { why = "mech is NULL"; return -1; }
+    if (m->numb_mech_spc <= 0 || m->nrxns <= 0 || m->mxprd <= 0) { why = "bad sizes"; return -1; }
+    if (!m->irr || !m->sc || !m->nreact || !m->nprdct || !m->ktype || !m->irxbits || !m->iorder || !m->rtdat) {
+        why = "missing table"; return -1;
+    }
+    if (m->nspecial_rxn != 0) { why = "SPECIAL rate operators not supported"; return -1; }
+    Tab t{m};
+    for (int r = 0; r < m->nrxns; ++r) {
+        if (m->nreact[r] < 0 || m->nreact[r] > 3 || m->nprdct[r] < 0 || m->nprdct[r] > m->mxprd) {
+            why = "bad reactant/product count in reaction " + std::to_string(r + 1); return -1;
+        }
+        for (int k = 0; k < m->nreact[r]; ++k)
+            if (t.irr(r, k) < 1 || t.irr(r, k) > m->numb_mech_spc) { why = "bad species id"; return -1; }
+        for (int k = 0; k < m->nprdct[r]; ++k)
+            if (t.irr(r, 3 + k) < 1 || t.irr(r, 3 + k) > m->numb_mech_spc) { why = "bad species id"; return -1; }
+    }
+    return 0;
+}
+
+template <class T>
+int dalloc(cmaq_ros3_ctx *c, T **p, size_t n)
+{
+    if (hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
+        return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc failed");
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech, const double *rtol,
+                              const double *atol, const cmaq_ros3_opts *opts)
+{
+    if (!out) return set_err(nullptr, CMAQ_ROS3_E_ARG, "out is NULL");
+    *out = nullptr;
+    std::string why;
+    if (validate(mech, why) != 0) return set_err(nullptr, CMAQ_ROS3_E_MECH, why);
+    Tab t{mech};
+    const unsigned long long h = structure_hash(t);
+    const MechKernels *mk = nullptr;
+    for (int i = 0; i < ros3_mech_registry_size; ++i)
+        if (ros3_mech_registry[i]->struct_hash == h && ros3_mech_registry[i]->ns == mech->numb_mech_spc &&
+            ros3_mech_registry[i]->nr == mech->nrxns)
+            mk = ros3_mech_registry[i];
+    if (!mk) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "no kernels built for this mechanism (structure hash %016llx)", h);
+        return set_err(nullptr, CMAQ_ROS3_E_MECH, buf);
+    }
+    if (mech->nrxns > kMaxRates) return set_err(nullptr, CMAQ_ROS3_E_MECH, "more reactions than the rates kernel stages");
+    cmaq_ros3_ctx *c = new cmaq_ros3_ctx();
+    c->mk = mk;
+    c->ns = mech->numb_mech_spc;
+    c->nr = mech->nrxns;
+    c->nphotab = mech->nphotab;
+    c->nhetero = mech->nhetero;
+    if (decode_rates(t, c->ops, c->mul, why) != 0) { delete c; return set_err(nullptr, CMAQ_ROS3_E_MECH, why); }
+    if ((int)c->mul.size() > kMaxMul) { delete c; return set_err(nullptr, CMAQ_ROS3_E_MECH, "too many multipliers"); }
+    // tolerances in solver order (rbsolver.F:434-441 indexes them by the solver species)
+    c->rtol_new.resize(c->ns);
+    c->atol_new.resize(c->ns);
+    for (int i = 0; i < c->ns; ++i) {
+        int old = mk->inew2old[i];
+        c->rtol_new[i] = rtol ? rtol[old] : 1.0e-3;
+        c->atol_new[i] = atol ? atol[old] : 1.0e-7;
+        if (!(c->rtol_new[i] > 0.0) || !(c->atol_new[i] > 0.0)) {
+            delete c; return set_err(nullptr, CMAQ_ROS3_E_ARG, "tolerances must be positive");
+        }
+    }
+    if (opts) {
+        if (opts->max_attempts > 0) c->max_attempts = opts->max_attempts;
+        c->refill_min = opts->refill_min;
+        c->waves_per_cu = opts->waves_per_cu;
+    }
+    int dev = (opts && opts->device >= 0) ? opts->device : -1;
+    if (dev < 0) {
+        if (hipGetDevice(&dev) != hipSuccess) { delete c; return set_err(nullptr, CMAQ_ROS3_E_HIP, "no HIP device"); }
+    }
+    c->device = dev;
+    int rc = 0;
+#define INIT_TRY(expr) do { hipError_t _e = (expr); if (_e != hipSuccess) { rc = set_err(c, CMAQ_ROS3_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); goto fail; } } while (0)
+    {
+        INIT_TRY(hipSetDevice(dev));
+        hipDeviceProp_t prop;
+        INIT_TRY(hipGetDeviceProperties(&prop, dev));
+        int per_cu = 0;
+        INIT_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)mk->solve_kernel, kSolveBlock, 0));
+        if (per_cu < 1) per_cu = 1;
+        const int wpb = kSolveBlock / kWave;
+        if (c->waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, c->waves_per_cu / wpb));
+        c->grid_blocks = prop.multiProcessorCount * per_cu;
+        c->waves = (int64_t)c->grid_blocks * wpb;
+        if (c->refill_min <= 0) c->refill_min = 16;
+        if (c->refill_min > kWave) c->refill_min = kWave;
+        if (dalloc(c, &c->d_ops, c->ops.size())) goto fail_nomem;
+        if (dalloc(c, &c->d_mul, std::max<size_t>(c->mul.size(), 1))) goto fail_nomem;
+        if (dalloc(c, &c->d_rtol, c->ns) || dalloc(c, &c->d_atol, c->ns)) goto fail_nomem;
+        if (dalloc(c, &c->d_ws, (size_t)c->waves * mk->nws * kWave)) goto fail_nomem;
+        if (dalloc(c, (char **)&c->d_small, 256)) goto fail_nomem;
+        c->d_queue = (unsigned long long *)c->d_small;
+        c->d_fail_cell = (int64_t *)((char *)c->d_small + 8);
+        c->d_fail_info = (double *)((char *)c->d_small + 16);
+        c->d_fail_flag = (int32_t *)((char *)c->d_small + 40);
+        c->d_fail_code = (int32_t *)((char *)c->d_small + 44);
+        INIT_TRY(hipMemcpy(c->d_ops, c->ops.data(), c->ops.size() * sizeof(RateOp), hipMemcpyHostToDevice));
+        if (!c->mul.empty())
+            INIT_TRY(hipMemcpy(c->d_mul, c->mul.data(), c->mul.size() * sizeof(RateMul), hipMemcpyHostToDevice));
+        INIT_TRY(hipMemcpy(c->d_rtol, c->rtol_new.data(), c->ns * sizeof(double), hipMemcpyHostToDevice));
+        INIT_TRY(hipMemcpy(c->d_atol, c->atol_new.data(), c->ns * sizeof(double), hipMemcpyHostToDevice));
+    }
+#undef INIT_TRY
+    *out = c;
+    return 0;
+fail_nomem:
+    rc = CMAQ_ROS3_E_NOMEM;
+fail:
+    cmaq_ros3_free(c);
+    return rc;
+}
+
+extern "C" void cmaq_ros3_free(cmaq_ros3_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipFree(c->d_ops); hipFree(c->d_mul); hipFree(c->d_rtol); hipFree(c->d_atol);
+    hipFree(c->d_ws); hipFree(c->d_small); hipFree(c->d_rki); hipFree(c->d_io);
+    delete c;
+}
+
+extern "C" const char *cmaq_ros3_last_error(const cmaq_ros3_ctx *c)
+{
+    return c ? c->err.c_str() : g_last_error.c_str();
+}
+
+extern "C" int cmaq_ros3_query(const cmaq_ros3_ctx *c, const char *key, int64_t *value)
+{
+    if (!c || !key || !value) return CMAQ_ROS3_E_ARG;
+    const MechKernels *k = c->mk;
+    if (!std::strcmp(key, "ischan")) *value = k->ns;
+    else if (!std::strcmp(key, "nnz_day")) *value = k->nnz_day;
+    else if (!std::strcmp(key, "nnz_night")) *value = k->nnz_night;
+    else if (!std::strcmp(key, "lu_ops_day")) *value = k->lu_ops_day;
+    else if (!std::strcmp(key, "lu_ops_night")) *value = k->lu_ops_night;
+    else if (!std::strcmp(key, "nrxns")) *value = k->nr;
+    else if (!std::strcmp(key, "numb_mech_spc")) *value = k->ns;
+    else if (!std::strcmp(key, "grid_waves")) *value = c->waves;
+    else if (!std::strcmp(key, "workspace_doubles_per_lane")) *value = k->nws;
+    else return CMAQ_ROS3_E_ARG;
+    return 0;
+}
+
+extern "C" int cmaq_ros3_species_order(const cmaq_ros3_ctx *c, int32_t *inew2old)
+{
+    if (!c || !inew2old) return CMAQ_ROS3_E_ARG;
+    std::memcpy(inew2old, c->mk->inew2old, sizeof(int32_t) * c->ns);
+    return 0;
+}
+
+static int launch_rates(cmaq_ros3_ctx *c, int64_t n, const double *temp, const double *pres,
+                        const double *h2o, const double *rj, const double *het, const double *sea,
+                        double *rki, hipStream_t s)
+{
+    RatesParams P;
+    P.ncells = n; P.nr = c->nr; P.nphotab = c->nphotab; P.nhetero = c->nhetero;
+    P.nmul = (int32_t)c->mul.size();
+    P.ops = c->d_ops; P.mul = c->d_mul;
+    P.temp = temp; P.pres = pres; P.h2o = h2o; P.rj = rj; P.het = het; P.sea = sea;
+    P.rki = rki; P.sunlit = nullptr;
+    int64_t blocks = (n + 255) / 256;
+    blocks = std::min<int64_t>(blocks, 65536);
+    hipLaunchKernelGGL((rates_kernel<kMaxRates, kMaxMul>), dim3((unsigned)blocks), dim3(256), 0, s, P);
+    HIP_TRY(c, hipGetLastError());
+    return 0;
+}
+
+extern "C" int cmaq_ros3_rates_device(cmaq_ros3_ctx *c, int64_t n, const double *temp, const double *pres,
+                                      const double *h2o, const double *rj, const double *het,
+                                      const double *sea, double *rki, void *stream)
+{
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (n < 0 || (n > 0 && (!temp || !pres || !h2o || !rki || (c->nphotab > 0 && !rj))))
+        return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    int rc = launch_rates(c, n, temp, pres, h2o, rj, het, sea, rki, s);
+    if (rc) return rc;
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int cmaq_ros3_run_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *temp,
+                                    const double *pres, const double *h2o, const double *rj,
+                                    const double *het, const double *sea, double *y, int32_t *nacc,
+                                    int32_t *nrej, cmaq_ros3_fail *fail, void *stream)
+{
+    if (fail) { fail->code = 0; fail->pad = 0; fail->cell = -1; fail->maxerr = fail->dt = fail->tnow = 0.0; }
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (n < 0 || !(chemstep > 0.0) || (n > 0 && (!temp || !pres || !h2o || !y || (c->nphotab > 0 && !rj))))
+        return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (c->rki_cells < n) {
+        hipFree(c->d_rki);
+        c->d_rki = nullptr;
+        c->rki_cells = 0;
+        if (hipMalloc((void **)&c->d_rki, (size_t)n * c->nr * sizeof(double)) != hipSuccess)
+            return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of rate constants failed");
+        c->rki_cells = n;
+    }
+    int rc = launch_rates(c, n, temp, pres, h2o, rj, het, sea, c->d_rki, s);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
+    SolveParams P;
+    P.ncells = n; P.chemstep = chemstep; P.rki = c->d_rki; P.y = y; P.nacc = nacc; P.nrej = nrej;
+    P.rtol = c->d_rtol; P.atol = c->d_atol; P.ws = c->d_ws; P.queue = c->d_queue;
+    P.fail_flag = c->d_fail_flag; P.fail_cell = c->d_fail_cell; P.fail_info = c->d_fail_info;
+    P.fail_code = c->d_fail_code; P.max_attempts = c->max_attempts; P.refill_min = c->refill_min;
+    void *args[] = {&P};
+    HIP_TRY(c, hipLaunchKernel(c->mk->solve_kernel, dim3(c->grid_blocks), dim3(kSolveBlock), args, 0, s));
+    unsigned char small[64];
+    HIP_TRY(c, hipMemcpyAsync(small, c->d_small, 64, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    int32_t flag, code;
+    int64_t cell;
+    double info[3];
+    std::memcpy(&cell, small + 8, 8);
+    std::memcpy(info, small + 16, 24);
+    std::memcpy(&flag, small + 40, 4);
+    std::memcpy(&code, small + 44, 4);
+    if (flag) {
+        if (fail) { fail->code = code; fail->cell = cell; fail->maxerr = info[0]; fail->dt = info[1]; fail->tnow = info[2]; }
+        char buf[200];
+        std::snprintf(buf, sizeof buf, "%s in cell %lld (MAXERR %.4e, DT %.4e min, TNOW %.4e min)",
+                      code == CMAQ_ROS3_E_CONVERGENCE ? "convergence failure" : "attempted-step limit",
+                      (long long)cell, info[0], info[1], info[2]);
+        return set_err(c, code, buf);
+    }
+    return 0;
+}
+
+extern "C" int cmaq_ros3_run(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *temp,
+                             const double *pres, const double *h2o, const double *rj, const double *het,
+                             const double *sea, double *y, int32_t *nacc, int32_t *nrej,
+                             cmaq_ros3_fail *fail)
+{
+    if (fail) { fail->code = 0; fail->pad = 0; fail->cell = -1; fail->maxerr = fail->dt = fail->tnow = 0.0; }
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (n < 0 || !(chemstep > 0.0) || (n > 0 && (!temp || !pres || !h2o || !y || (c->nphotab > 0 && !rj))))
+        return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t nd = (size_t)n * sizeof(double);
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t sizes[9] = {nd, nd, nd, nd * c->nphotab, het ? nd * c->nhetero : 0, sea ? nd : 0,
+                             nd * c->ns, (size_t)n * 4, (size_t)n * 4};
+    size_t need = 0;
+    for (size_t b : sizes) need += al(b);
+    if (c->io_bytes < need) {
+        hipFree(c->d_io);
+        c->d_io = nullptr;
+        c->io_bytes = 0;
+        if (hipMalloc(&c->d_io, need) != hipSuccess) return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of I/O buffers failed");
+        c->io_bytes = need;
+    }
+    char *bufs[9];
+    char *p = (char *)c->d_io;
+    for (int i = 0; i < 9; ++i) { bufs[i] = sizes[i] ? p : nullptr; p += al(sizes[i]); }
+    double *d_t = (double *)bufs[0], *d_p = (double *)bufs[1], *d_h = (double *)bufs[2];
+    double *d_rj = (double *)bufs[3], *d_het = (double *)bufs[4], *d_sea = (double *)bufs[5];
+    double *d_y = (double *)bufs[6];
+    int32_t *d_na = (int32_t *)bufs[7], *d_nr = (int32_t *)bufs[8];
+    HIP_TRY(c, hipMemcpy(d_t, temp, nd, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(d_p, pres, nd, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(d_h, h2o, nd, hipMemcpyHostToDevice));
+    if (c->nphotab) HIP_TRY(c, hipMemcpy(d_rj, rj, nd * c->nphotab, hipMemcpyHostToDevice));
+    if (het && c->nhetero) HIP_TRY(c, hipMemcpy(d_het, het, nd * c->nhetero, hipMemcpyHostToDevice));
+    if (sea) HIP_TRY(c, hipMemcpy(d_sea, sea, nd, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(d_y, y, nd * c->ns, hipMemcpyHostToDevice));
+    int rc = cmaq_ros3_run_device(c, n, chemstep, d_t, d_p, d_h, d_rj, d_het, d_sea, d_y, d_na, d_nr, fail, nullptr);
+    if (rc != 0 && rc != CMAQ_ROS3_E_CONVERGENCE && rc != CMAQ_ROS3_E_STEPS) return rc;
+    HIP_TRY(c, hipMemcpy(y, d_y, nd * c->ns, hipMemcpyDeviceToHost));
+    if (nacc) HIP_TRY(c, hipMemcpy(nacc, d_na, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (nrej) HIP_TRY(c, hipMemcpy(nrej, d_nr, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return rc;
+}

@@ -1,0 +1,20 @@
+// Record that each generated mechanism translation unit exports so that
+// cmaq_ros3_init can select the kernels matching the tables it is given.
+#pragma once
+
+#include <stdint.h>
+
+namespace ros3 {
+
+struct MechKernels {
+    const char *name;
+    unsigned long long struct_hash;   // FNV-1a of the structural tables
+    int ns, nr, nnz_day, lu_ops_day, nnz_night, lu_ops_night, nws;
+    const void *solve_kernel;         // host stub of solve_kernel
+    const int32_t *inew2old;          // solver order -> mechanism order (0-based)
+};
+
+}  // namespace ros3
+
+extern "C" const ros3::MechKernels *const ros3_mech_registry[];
+extern "C" const int ros3_mech_registry_size;

@@ -1,0 +1,149 @@
+"""Seeded synthetic cell inputs for the ROS3 workloads.
+
+Every value is a pure function of (seed, field, cell index) through a
+splitmix64 hash, so any subset of cells reproduces exactly (the CPU baseline
+times a subset of the same cells the GPU integrates).
+
+Distributions (per cell):
+  T ~ U(250, 310) K;  P ~ U(500, 1013) hPa, passed in atm (hPa*100/101325);
+  H2O log-uniform on [1e3, 3e4] ppm;
+  every species log-uniform on [1e-6, 1e2] ppb, except O3 ~ U(20, 80) ppb and
+  CO ~ U(80, 300) ppb; concentrations are passed in ppm;
+  species whose chemical lifetime is shorter than the solver's minimum step
+  DTMIN = 1e-8 min (O1D in both mechanisms, ~2e-11 min) start at 1e-6 ppb:
+  for them ROS3's embedded error estimate tends to y0/2 (it is not damped as
+  h*lambda -> -inf), so a larger start value makes RBSOLVER reject down to
+  DTMIN and stop (rbsolver.F:528-583) -- with the plain distribution the
+  reference fails in ~58 % of cells;
+  J = clear-sky profile (CLEAR_SKY_J, 1/min) * cos(zenith), cos(zenith) ~ U(0,1),
+  with 30 % of cells at night (J = 0);
+  heterogeneous rates and the sea-water fraction are 0.
+Workload variants:
+  regional  - cells ordered layer-major over nlay layers, J * exp(-k/20);
+  mixed     - 40 % clean day, 30 % night, 30 % near-source with NO and NO2
+              log-uniform on [10, 500] ppb and the VOC species x10.
+"""
+
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix(x):
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & _M64
+    z = x
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _M64
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _M64
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed, field, cells):
+    """U[0,1) from (seed, field, cell index); cells is an int array."""
+    with np.errstate(over="ignore"):
+        key = _splitmix(np.uint64(seed) * np.uint64(0x100000001B3) + np.uint64(field))
+        h = _splitmix(key ^ (np.asarray(cells, dtype=np.uint64) * np.uint64(0xD6E8FEB86659FD93)))
+    return (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def _logu(u, lo, hi):
+    return np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))
+
+
+# Clear-sky, overhead-sun surface photolysis frequencies (1/s) assumed for the
+# synthetic workloads; the reference repository holds no J table, these are
+# representative mid-latitude summer values per CSQY table name.
+_CLEAR_SKY_J_PER_S = {
+    # CB6r5 tables
+    "NO2_IUPAC10": 8.0e-3, "O3_O3P_IUPAC10": 4.0e-4, "O3_O1D_IUPAC10": 3.0e-5,
+    "H2O2_IUPAC10": 7.0e-6, "NO3NO2_06": 1.8e-1, "NO3NO_06": 2.2e-2,
+    "N2O5_IUPAC10": 3.0e-5, "HONO_IUPAC10": 1.5e-3, "HNO3_IUPAC10": 6.0e-7,
+    "PNA_IUPAC10": 5.0e-6, "PAN_IUPAC10": 4.0e-7, "MEPX_IUPAC10": 5.0e-6,
+    "NTR_IUPAC10": 1.0e-6, "FORM_R_IUPAC13": 3.0e-5, "FORM_M_IUPAC13": 4.5e-5,
+    "ALD2_R_IUPAC13": 4.0e-6, "ALDX_R_IUPAC13": 1.5e-5, "GLYD_IUPAC13": 3.0e-6,
+    "GLY_R_IUPAC13": 3.0e-5, "MGLY_IUPAC10": 1.2e-4, "KET_IUPAC10": 1.0e-6,
+    "ACET_IUPAC10": 5.0e-7, "ISPD": 1.0e-5, "HPALD": 2.0e-4, "CL2_IUPAC04": 2.0e-3,
+    "HOCL_IUPAC04": 2.5e-4, "FMCL_IUPAC04": 1.0e-5, "CLNO2_IUPAC13": 3.0e-4,
+    "CLONO2_1": 3.0e-5, "CLONO2_2": 1.0e-5, "IC3ONO2": 5.0e-6, "ACRO_09": 2.0e-6,
+    # CRACMM2 tables
+    "O3O3P_NASA06": 4.0e-4, "O3O1D_NASA06": 3.0e-5, "H2O2_RACM2": 7.0e-6,
+    "NO2_RACM2": 8.0e-3, "NO3NO_RACM2": 2.2e-2, "NO3NO2_RACM2": 1.8e-1,
+    "HONO_RACM2": 1.5e-3, "HNO3_RACM2": 6.0e-7, "HNO4_RACM2": 5.0e-6,
+    "HCHO_MOL_JPL19": 4.5e-5, "HCHO_RAD_JPL19": 3.0e-5, "CH3CHO_RACM2": 4.0e-6,
+    "ALD_JPL19": 1.5e-5, "CH3COCH3A_JPL19": 5.0e-7, "CH3COCH3B_JPL19": 2.0e-7,
+    "UALD_RACM2": 1.0e-5, "MEK_JGR19": 1.0e-6, "KET_JGR19": 1.0e-6, "HKET_RACM2": 2.0e-6,
+    "MACR_RACM2": 3.0e-6, "MVK_RACM2": 5.0e-6, "GLYH2_RACM2": 1.0e-5, "GLYF_RACM2": 2.0e-5,
+    "GLYHX_RACM2": 3.0e-5, "MGLY_RACM2": 1.2e-4, "BALD_RACM2": 5.0e-5, "OP1_RACM2": 5.0e-6,
+    "PAA_RACM2": 1.0e-6, "ONIT_RACM2": 1.0e-6, "PAN1_RACM2": 3.0e-7, "PAN2_RACM2": 1.0e-7,
+}
+
+VOC_SPECIES = {
+    "CB6R5_AE7_AQ": ["PAR", "OLE", "IOLE", "ETH", "ETHA", "ETOH", "MEOH", "FORM", "ALD2", "ALDX",
+                     "ISOP", "TERP", "APIN", "TOL", "XYLMN", "BENZENE", "NAPH", "PRPA", "ETHY", "KET",
+                     "ACET", "SESQ", "SOAALK", "FORM_PRIMARY", "ALD2_PRIMARY", "BUTADIENE13",
+                     "ACROLEIN", "TOLU"],
+    "CRACMM2": ["ETH", "HC3", "HC5", "HC10", "ETE", "OLT", "OLI", "ACE", "BEN", "TOL", "XYL", "EBZ",
+                "ISO", "API", "LIM", "HCHO", "ACD", "ALD", "ACT", "MEK", "KET", "MOH", "EOH", "ROH",
+                "SESQ", "STY", "NAPH", "FURAN", "BDE13", "ACRO", "PROG"],
+}
+
+# species with a chemical lifetime below DTMIN at any atmospheric state
+SUB_DTMIN_SPECIES = {"CB6R5_AE7_AQ": ["O1D"], "CRACMM2": ["O1D"]}
+
+F_T, F_P, F_H2O, F_COSZ, F_NIGHT, F_REGIME, F_NO, F_NO2, F_SPC = 1, 2, 3, 4, 5, 6, 7, 8, 100
+
+
+def clear_sky_j(mech):
+    """Clear-sky J profile in 1/min, one value per PHOTAB entry."""
+    return np.array([_CLEAR_SKY_J_PER_S[n] * 60.0 for n in mech["photab"]])
+
+
+def make_cells(mech, cells, seed=0, workload="clean", nlay_cells=None, svoc_ppb=None):
+    """Inputs for the given cell indices (int array).
+
+    workload: 'clean' (config-0 distributions), 'regional' (plus J decay with
+    layer; nlay_cells = cells per layer) or 'mixed' (three regimes).
+    svoc_ppb: optional (lo, hi) log-uniform range for species whose name
+    starts with 'VROC' (the larger-mechanism workload).
+    Returns a dict of float64 arrays in the C-ABI layouts.
+    """
+    cells = np.asarray(cells, dtype=np.int64)
+    n = cells.size
+    names = mech["chemistry_spc"]
+    ns = len(names)
+    temp = 250.0 + 60.0 * uniform(seed, F_T, cells)
+    pres = (500.0 + 513.0 * uniform(seed, F_P, cells)) * 100.0 / 101325.0
+    h2o = _logu(uniform(seed, F_H2O, cells), 1.0e3, 3.0e4)
+    y = np.empty((ns, n))
+    fast = set(SUB_DTMIN_SPECIES.get(mech["mechname"], []))
+    for i, nm in enumerate(names):
+        u = uniform(seed, F_SPC + i, cells)
+        if nm in fast:
+            v = np.full(n, 1.0e-6)
+        elif nm == "O3":
+            v = 20.0 + 60.0 * u
+        elif nm == "CO":
+            v = 80.0 + 220.0 * u
+        elif svoc_ppb is not None and nm.startswith("VROC"):
+            v = _logu(u, *svoc_ppb)
+        else:
+            v = _logu(u, 1.0e-6, 1.0e2)
+        y[i] = v * 1.0e-3
+    cosz = uniform(seed, F_COSZ, cells)
+    night = uniform(seed, F_NIGHT, cells) < 0.30
+    if workload == "mixed":
+        reg = uniform(seed, F_REGIME, cells)
+        night = (reg >= 0.40) & (reg < 0.70)
+        source = reg >= 0.70
+        idx = {nm: i for i, nm in enumerate(names)}
+        for nm, fld in (("NO", F_NO), ("NO2", F_NO2)):
+            y[idx[nm], source] = _logu(uniform(seed, fld, cells[source]), 10.0, 500.0) * 1.0e-3
+        for nm in VOC_SPECIES.get(mech["mechname"], []):
+            if nm in idx:
+                y[idx[nm], source] *= 10.0
+    scale = np.where(night, 0.0, cosz)
+    if workload == "regional":
+        k = cells // int(nlay_cells)
+        scale = scale * np.exp(-k / 20.0)
+    rj = clear_sky_j(mech)[:, None] * scale[None, :]
+    return {"temp": temp, "pres": pres, "h2o": h2o, "rj": np.ascontiguousarray(rj),
+            "y": np.ascontiguousarray(y), "night": night}

@@ -1,0 +1,233 @@
+"""Python host side of the gfx950 ROS3 solver (ctypes over libcmaq_ros3.so).
+
+Mirrors the reference call sequence of the ros3 CHEM driver
+(CCTM/src/gas/ros3/rbdriver.F): ``Ros3Solver(...)`` plays RBINIT + RBSPARSE
+(rbinit.F:26, rbsparse.F:26) and ``run`` plays CALC_RCONST + RBSOLVER over all
+cells (rbdriver.F:440-677), with the reference's units (ppm, minutes, 1/min,
+atm) and "cell fastest" layouts.  Failures raise ``Ros3Error`` carrying the
+code and the failing cell instead of stopping the process (the reference
+calls M3EXIT, rbsolver.F:583).
+
+The native library is required: nothing here falls back to a CPU path.
+"""
+
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcmaq_ros3.so")
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+E_CODES = {-1: "ARG", -2: "MECH", -3: "HIP", -4: "CONVERGENCE", -5: "STEPS", -6: "NOMEM"}
+
+
+class Ros3Error(RuntimeError):
+    def __init__(self, code, msg, fail=None):
+        super().__init__(f"cmaq_ros3 error {code} ({E_CODES.get(code, '?')}): {msg}")
+        self.code = code
+        self.fail = fail
+
+
+class _Mech(C.Structure):
+    _fields_ = [
+        ("numb_mech_spc", C.c_int32), ("nrxns", C.c_int32), ("mxprd", C.c_int32), ("kunits", C.c_int32),
+        ("irr", _i32p), ("sc", _f64p), ("nreact", _i32p), ("nprdct", _i32p), ("ktype", _i32p),
+        ("irxbits", _i32p), ("iorder", _i32p), ("rtdat", _f64p), ("nfalloff", C.c_int32),
+        ("irrfall", _i32p), ("rfdat", _f64p), ("nmphot", C.c_int32), ("nphotab", C.c_int32),
+        ("iph", _i32p), ("mhetero", C.c_int32), ("nhetero", C.c_int32), ("ihetero", _i32p),
+        ("nwm", C.c_int32), ("nww", C.c_int32), ("nwo2", C.c_int32), ("nwn2", C.c_int32),
+        ("nwch4", C.c_int32), ("nwh2", C.c_int32),
+        ("nrxwm", _i32p), ("nrxww", _i32p), ("nrxwo2", _i32p), ("nrxwn2", _i32p),
+        ("nrxwch4", _i32p), ("nrxwh2", _i32p),
+        ("atm_air", C.c_double), ("atm_o2", C.c_double), ("atm_n2", C.c_double),
+        ("atm_ch4", C.c_double), ("atm_h2", C.c_double), ("nspecial_rxn", C.c_int32),
+    ]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("max_attempts", C.c_int32), ("waves_per_cu", C.c_int32),
+                ("refill_min", C.c_int32)]
+
+
+class _Fail(C.Structure):
+    _fields_ = [("code", C.c_int32), ("pad", C.c_int32), ("cell", C.c_int64), ("maxerr", C.c_double),
+                ("dt", C.c_double), ("tnow", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the native library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m cmaq_amd.build` first")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.cmaq_ros3_init.argtypes = [C.POINTER(vp), C.POINTER(_Mech), _f64p, _f64p, C.POINTER(_Opts)]
+        L.cmaq_ros3_run.argtypes = [vp, C.c_int64, C.c_double] + [_f64p] * 7 + [_i32p, _i32p, C.POINTER(_Fail)]
+        L.cmaq_ros3_run_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 9 + [C.POINTER(_Fail), vp]
+        L.cmaq_ros3_rates_device.argtypes = [vp, C.c_int64] + [vp] * 7 + [vp]
+        L.cmaq_ros3_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
+        L.cmaq_ros3_species_order.argtypes = [vp, _i32p]
+        L.cmaq_ros3_last_error.argtypes = [vp]
+        L.cmaq_ros3_last_error.restype = C.c_char_p
+        L.cmaq_ros3_free.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def load_mechanism(name_or_path):
+    """Mechanism tables (RXNS_DATA) as a dict; name is e.g. 'cb6r5_ae7_aq'."""
+    path = name_or_path
+    if not os.path.exists(path):
+        path = os.path.join(HERE, "mechs", f"{name_or_path}.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def pack_mechanism(m):
+    """cmaq_ros3_mech with Fortran column-major arrays, as CCTM would pass them."""
+    keep = []
+
+    def i32(a, order="F"):
+        arr = np.asarray(a, dtype=np.int32)
+        arr = np.ascontiguousarray(arr.T) if (arr.ndim == 2 and order == "F") else np.ascontiguousarray(arr)
+        if arr.size == 0:
+            arr = np.zeros(1, np.int32)
+        keep.append(arr)
+        return arr.ctypes.data_as(_i32p)
+
+    def f64(a, order="F"):
+        arr = np.asarray(a, dtype=np.float64)
+        arr = np.ascontiguousarray(arr.T) if (arr.ndim == 2 and order == "F") else np.ascontiguousarray(arr)
+        keep.append(arr)
+        return arr.ctypes.data_as(_f64p)
+
+    s = _Mech()
+    s.numb_mech_spc, s.nrxns, s.mxprd, s.kunits = m["numb_mech_spc"], m["nrxns"], m["mxprd"], m["kunits"]
+    # JSON holds IRR/SC reaction-major ([r][slot]); Fortran memory is column-major
+    s.irr, s.sc = i32(m["irr"]), f64(m["sc"])
+    s.nreact, s.nprdct, s.ktype = i32(m["nreact"]), i32(m["nprdct"]), i32(m["ktype"])
+    s.irxbits, s.iorder = i32(m["irxbits"]), i32(m["iorder"])
+    # RTDAT(3, NRXNS): JSON rtdat[r] = [k1, k2, k3] is already the Fortran memory order
+    s.rtdat = f64(m["rtdat"], order="C")
+    s.nfalloff, s.irrfall, s.rfdat = m["nfalloff"], i32(m["irrfall"]), f64(m["rfdat"], order="C")
+    s.nmphot, s.nphotab, s.iph = m["nmphot"], m["nphotab"], i32(m["iph"])
+    s.mhetero, s.nhetero, s.ihetero = m["mhetero"], m["nhetero"], i32(m["ihetero"])
+    for key, nkey in (("nrxwm", "nwm"), ("nrxww", "nww"), ("nrxwo2", "nwo2"), ("nrxwn2", "nwn2"),
+                      ("nrxwch4", "nwch4"), ("nrxwh2", "nwh2")):
+        setattr(s, key, i32(m[key]))
+        setattr(s, nkey, len(m[key]))
+    s.atm_air, s.atm_o2, s.atm_n2, s.atm_ch4, s.atm_h2 = (m["atm_air"], m["atm_o2"], m["atm_n2"],
+                                                          m["atm_ch4"], m["atm_h2"])
+    s.nspecial_rxn = 0
+    return s, keep
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class Ros3Solver:
+    """One solver context (RBINIT + RBSPARSE) for one mechanism on one GPU."""
+
+    def __init__(self, mech="cb6r5_ae7_aq", rtol=1.0e-3, atol=1.0e-7, device=-1,
+                 max_attempts=0, waves_per_cu=0, refill_min=0):
+        self.mech = load_mechanism(mech) if isinstance(mech, str) else mech
+        self.ns = self.mech["numb_mech_spc"]
+        self.nr = self.mech["nrxns"]
+        self.nphotab = self.mech["nphotab"]
+        self.nhetero = self.mech["nhetero"]
+        s, keep = pack_mechanism(self.mech)
+        rt = np.ascontiguousarray(np.broadcast_to(np.asarray(rtol, np.float64), (self.ns,)))
+        at = np.ascontiguousarray(np.broadcast_to(np.asarray(atol, np.float64), (self.ns,)))
+        self.rtol, self.atol = rt, at
+        o = _Opts(device, max_attempts, waves_per_cu, refill_min)
+        ctx = C.c_void_p()
+        rc = lib().cmaq_ros3_init(C.byref(ctx), C.byref(s), rt.ctypes.data_as(_f64p),
+                                  at.ctypes.data_as(_f64p), C.byref(o))
+        del keep
+        if rc != 0:
+            raise Ros3Error(rc, lib().cmaq_ros3_last_error(None).decode())
+        self._ctx = ctx
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            lib().cmaq_ros3_free(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self, rc, fail=None):
+        return Ros3Error(rc, lib().cmaq_ros3_last_error(self._ctx).decode(), fail)
+
+    def query(self, key):
+        v = C.c_int64()
+        rc = lib().cmaq_ros3_query(self._ctx, key.encode(), C.byref(v))
+        if rc != 0:
+            raise self._err(rc)
+        return v.value
+
+    def species_order(self):
+        out = np.zeros(self.ns, np.int32)
+        lib().cmaq_ros3_species_order(self._ctx, out.ctypes.data_as(_i32p))
+        return out
+
+    def run(self, chemstep, temp, pres_atm, h2o_ppm, rj, y, het=None, seawater=None):
+        """Host arrays (numpy float64); returns (y_new, n_accepted, n_rejected)."""
+        temp = np.ascontiguousarray(temp, np.float64)
+        n = temp.size
+        pres_atm = np.ascontiguousarray(pres_atm, np.float64)
+        h2o_ppm = np.ascontiguousarray(h2o_ppm, np.float64)
+        rj = np.ascontiguousarray(rj, np.float64).reshape(self.nphotab, n)
+        yy = np.array(y, np.float64, copy=True, order="C").reshape(self.ns, n)
+        het_p = None if het is None else np.ascontiguousarray(het, np.float64).reshape(self.nhetero, n)
+        sw_p = None if seawater is None else np.ascontiguousarray(seawater, np.float64)
+        na, nj = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        f = _Fail()
+        p = lambda a: None if a is None else a.ctypes.data_as(_f64p)
+        rc = lib().cmaq_ros3_run(self._ctx, n, chemstep, p(temp), p(pres_atm), p(h2o_ppm), p(rj),
+                                 p(het_p), p(sw_p), p(yy), na.ctypes.data_as(_i32p), nj.ctypes.data_as(_i32p),
+                                 C.byref(f))
+        if rc != 0:
+            raise self._err(rc, {"cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow})
+        return yy, na, nj
+
+    def run_device(self, chemstep, temp, pres_atm, h2o_ppm, rj, y, nacc=None, nrej=None,
+                   het=None, seawater=None, stream=None):
+        """torch CUDA tensors (float64; nacc/nrej int32); y updated in place."""
+        import torch
+        n = temp.numel()
+        for t in (temp, pres_atm, h2o_ppm, rj, y):
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+                raise ValueError("inputs must be contiguous float64 CUDA tensors")
+        if y.numel() != self.ns * n or rj.numel() != self.nphotab * n:
+            raise ValueError("shape mismatch")
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_ros3_run_device(self._ctx, n, chemstep, _ptr(temp), _ptr(pres_atm), _ptr(h2o_ppm),
+                                        _ptr(rj), _ptr(het), _ptr(seawater), _ptr(y), _ptr(nacc), _ptr(nrej),
+                                        C.byref(f), C.c_void_p(stream))
+        if rc != 0:
+            raise self._err(rc, {"cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow})
+
+    def rates_device(self, temp, pres_atm, h2o_ppm, rj, rki, het=None, seawater=None, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        rc = lib().cmaq_ros3_rates_device(self._ctx, temp.numel(), _ptr(temp), _ptr(pres_atm), _ptr(h2o_ppm),
+                                          _ptr(rj), _ptr(het), _ptr(seawater), _ptr(rki), C.c_void_p(stream))
+        if rc != 0:
+            raise self._err(rc)

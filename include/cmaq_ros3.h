@@ -1,0 +1,158 @@
+/*
+ * cmaq_ros3.h -- C ABI of the MI355X (gfx950) ROS3 gas-chemistry solver.
+ *
+ * Drop-in for the per-block loop of CMAQ's ros3 CHEM driver
+ * (CCTM/src/gas/ros3/rbdriver.F:440-677), i.e. for
+ *     CALL CALC_RCONST(...)   (RXNS_FUNC_MODULE.F90:204, rbdriver.F:513)
+ *     CALL RBSOLVER(...)      (rbsolver.F:26, rbdriver.F:570)
+ * over every cell of a process's sub-domain in one call.  Symbolic set-up
+ * (RBINIT rbinit.F:26 + RBSPARSE rbsparse.F:26) happens in cmaq_ros3_init.
+ *
+ * Units and layouts follow the reference exactly:
+ *   concentrations ppm, time minutes, J-values and heterogeneous rates 1/min,
+ *   pressure atm, water vapour ppm, temperature K;
+ *   every per-cell array is "cell fastest" (Fortran (cell, index) order):
+ *   a[k * ncells + cell];  species in mechanism (CHEMISTRY_SPC) order.
+ * Step control is per cell: the result for a cell equals RBSOLVER run on a
+ * block holding that cell alone (NUMCELLS = 1), which decides day/night per
+ * cell (any J > 0) instead of per 50-cell block.
+ *
+ * All functions return 0 on success or a negative CMAQ_ROS3_E_* code; they
+ * never exit the process (the reference calls M3EXIT, rbsolver.F:583).
+ */
+#ifndef CMAQ_ROS3_H
+#define CMAQ_ROS3_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CMAQ_ROS3_OK              0
+#define CMAQ_ROS3_E_ARG          -1  /* bad argument / size                         */
+#define CMAQ_ROS3_E_MECH         -2  /* unsupported table feature or no kernel built */
+#define CMAQ_ROS3_E_HIP          -3  /* HIP runtime error                           */
+#define CMAQ_ROS3_E_CONVERGENCE  -4  /* DT fell below DTMIN (rbsolver.F:528-583)     */
+#define CMAQ_ROS3_E_STEPS        -5  /* attempted-step limit per cell exceeded       */
+#define CMAQ_ROS3_E_NOMEM        -6  /* device allocation failed                    */
+
+/*
+ * Mechanism tables exactly as MODULE RXNS_DATA holds them
+ * (MECHS/<mech>/RXNS_DATA_MODULE.F90).  Pointers refer to Fortran arrays
+ * passed with C_LOC; 2-D Fortran arrays are passed as they lie in memory
+ * (column-major), so IRR(NRXNS, MXPRD+3) arrives as irr[slot*nrxns + r].
+ * Ids inside the arrays stay 1-based.
+ */
+typedef struct cmaq_ros3_mech {
+    int32_t numb_mech_spc;      /* NUMB_MECH_SPC                                  */
+    int32_t nrxns;              /* NRXNS                                          */
+    int32_t mxprd;              /* MXPRD                                          */
+    int32_t kunits;             /* KUNITS                                         */
+    const int32_t *irr;         /* IRR(NRXNS, MXPRD+3)                            */
+    const double  *sc;          /* SC(NRXNS, MXPRD)                               */
+    const int32_t *nreact;      /* NREACT(NRXNS)                                  */
+    const int32_t *nprdct;      /* NPRDCT(NRXNS)                                  */
+    const int32_t *ktype;       /* KTYPE(NRXNS)                                   */
+    const int32_t *irxbits;     /* IRXBITS(NRXNS)                                 */
+    const int32_t *iorder;      /* IORDER(NRXNS)                                  */
+    const double  *rtdat;       /* RTDAT(3, NRXNS)                                */
+    int32_t nfalloff;           /* NFALLOFF                                       */
+    const int32_t *irrfall;     /* IRRFALL(NFALLOFF)                              */
+    const double  *rfdat;       /* RFDAT(5, NFALLOFF)                             */
+    int32_t nmphot;             /* NMPHOT                                         */
+    int32_t nphotab;            /* NPHOTAB                                        */
+    const int32_t *iph;         /* IPH(NMPHOT, 3)                                 */
+    int32_t mhetero;            /* MHETERO                                        */
+    int32_t nhetero;            /* NHETERO                                        */
+    const int32_t *ihetero;     /* IHETERO(MHETERO, 2)                            */
+    int32_t nwm, nww, nwo2, nwn2, nwch4, nwh2;
+    const int32_t *nrxwm, *nrxww, *nrxwo2, *nrxwn2, *nrxwch4, *nrxwh2;
+    double atm_air, atm_o2, atm_n2, atm_ch4, atm_h2;
+    int32_t nspecial_rxn;       /* must be 0 (SPECIAL_RATES not supported)        */
+} cmaq_ros3_mech;
+
+typedef struct cmaq_ros3_opts {
+    int32_t device;             /* HIP device ordinal; -1 = current device        */
+    int32_t max_attempts;       /* per cell and call; 0 = 1000000                 */
+    int32_t waves_per_cu;       /* resident waves per CU; 0 = autotuned / max     */
+    int32_t refill_min;         /* lanes that must be idle before a wave refills;
+                                   0 = autotuned                                  */
+} cmaq_ros3_opts;
+
+typedef struct cmaq_ros3_fail {
+    int32_t code;               /* CMAQ_ROS3_E_CONVERGENCE / _STEPS, or 0          */
+    int32_t pad;
+    int64_t cell;               /* failing cell (0-based), -1 if none              */
+    double maxerr;              /* MAXERR of the last rejected step                */
+    double dt;                  /* DT after the last reduction, minutes            */
+    double tnow;                /* elapsed minutes in the sync step                */
+} cmaq_ros3_fail;
+
+typedef struct cmaq_ros3_ctx cmaq_ros3_ctx;
+
+/*
+ * RBINIT + RBSPARSE replacement.  Validates the tables, derives the solver
+ * species order and sparse pattern, and selects the generated gfx950 kernels
+ * built for this mechanism (matched by a hash of the structural tables).
+ * rtol/atol: per species, mechanism order (rbinit.F:257-262 sets them all to
+ * RB_RTOL / RB_ATOL); NULL selects the defaults 1e-3 and 1e-7 ppm
+ * (util/util/RUNTIME_VARS.F:177-178).  opts may be NULL.
+ */
+int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech,
+                   const double *rtol, const double *atol,
+                   const cmaq_ros3_opts *opts);
+
+/*
+ * CALC_RCONST + RBSOLVER over ncells independent cells for one chemistry
+ * step of chemstep_min minutes (rbdriver.F:385 CHEMSTEP).  Host pointers;
+ * synchronous.  rj [nphotab][ncells], het [nhetero][ncells] (may be NULL =
+ * 0), seawater [ncells] (may be NULL = 0), y [numb_mech_spc][ncells] in/out.
+ * n_accepted / n_rejected (may be NULL) receive the per-cell step counts that
+ * the reference's rbstats build reports (rbsolver.F:462-464, 521-524).
+ */
+int cmaq_ros3_run(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min,
+                  const double *temp_k, const double *pres_atm,
+                  const double *h2o_ppm, const double *rj_per_min,
+                  const double *het_per_min, const double *seawater,
+                  double *y_ppm, int32_t *n_accepted, int32_t *n_rejected,
+                  cmaq_ros3_fail *fail);
+
+/*
+ * Same as cmaq_ros3_run with every array already in device memory (HBM),
+ * enqueued on `stream` (a hipStream_t; NULL = default stream).  Returns after
+ * the stream has finished so that failures are reported.
+ */
+int cmaq_ros3_run_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min,
+                         const double *temp_k, const double *pres_atm,
+                         const double *h2o_ppm, const double *rj_per_min,
+                         const double *het_per_min, const double *seawater,
+                         double *y_ppm, int32_t *n_accepted, int32_t *n_rejected,
+                         cmaq_ros3_fail *fail, void *stream);
+
+/*
+ * CALC_RCONST alone (RXNS_FUNC_MODULE.F90:204-1116) on device arrays:
+ * rki [nrxns][ncells] in ppm-min units.  Day/night is decided per cell.
+ */
+int cmaq_ros3_rates_device(cmaq_ros3_ctx *ctx, int64_t ncells,
+                           const double *temp_k, const double *pres_atm,
+                           const double *h2o_ppm, const double *rj_per_min,
+                           const double *het_per_min, const double *seawater,
+                           double *rki, void *stream);
+
+/* Sizes of the selected pattern: key in {"ischan","nnz_day","nnz_night",
+ * "lu_ops_day","lu_ops_night","nrxns","numb_mech_spc","grid_waves"}. */
+int cmaq_ros3_query(const cmaq_ros3_ctx *ctx, const char *key, int64_t *value);
+
+/* Solver species order: inew2old[new] = old (both 0-based), numb_mech_spc entries. */
+int cmaq_ros3_species_order(const cmaq_ros3_ctx *ctx, int32_t *inew2old);
+
+/* Message for the last failure on this context (static storage, never NULL). */
+const char *cmaq_ros3_last_error(const cmaq_ros3_ctx *ctx);
+
+void cmaq_ros3_free(cmaq_ros3_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CMAQ_ROS3_H */

@@ -1,0 +1,232 @@
+"""ctypes wrapper of the CPU oracle (liboracle_ros3.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's CPU-baseline leg, always as the checker / reported baseline,
+never as the product path.  The restated reference functions live in
+ros3_oracle.c (each cites the Fortran it follows).
+"""
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_ros3.so")
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class _Tables(C.Structure):
+    _fields_ = [
+        ("ns", C.c_int32), ("nr", C.c_int32), ("mxprd", C.c_int32), ("kunits", C.c_int32),
+        ("irr", _i32p), ("sc", _f64p),
+        ("nreact", _i32p), ("nprdct", _i32p), ("ktype", _i32p), ("irxbits", _i32p), ("iorder", _i32p),
+        ("rtdat", _f64p), ("nfalloff", C.c_int32), ("irrfall", _i32p), ("rfdat", _f64p),
+        ("nmphot", C.c_int32), ("nphotab", C.c_int32), ("iph", _i32p),
+        ("mhetero", C.c_int32), ("nhetero", C.c_int32), ("ihetero", _i32p),
+        ("nwm", C.c_int32), ("nww", C.c_int32), ("nwo2", C.c_int32), ("nwn2", C.c_int32),
+        ("nwch4", C.c_int32), ("nwh2", C.c_int32),
+        ("nrxwm", _i32p), ("nrxww", _i32p), ("nrxwo2", _i32p), ("nrxwn2", _i32p),
+        ("nrxwch4", _i32p), ("nrxwh2", _i32p),
+        ("atm_air", C.c_double), ("atm_o2", C.c_double), ("atm_n2", C.c_double),
+        ("atm_ch4", C.c_double), ("atm_h2", C.c_double),
+    ]
+
+
+class _Fail(C.Structure):
+    _fields_ = [("cell", C.c_int64), ("maxerr", C.c_double), ("dt", C.c_double), ("tnow", C.c_double)]
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle_ros3.so"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp = C.c_void_p
+        L.om_create.restype = vp
+        L.om_create.argtypes = [C.POINTER(_Tables), C.c_char_p, C.c_int]
+        L.om_destroy.argtypes = [vp]
+        for name in ("om_ischan",):
+            getattr(L, name).argtypes = [vp]
+        for name in ("om_iarray", "om_nuserat", "om_nlu_ops"):
+            getattr(L, name).argtypes = [vp, C.c_int]
+        L.om_inew2old.argtypes = [vp, _i32p]
+        L.om_nkuserat.argtypes = [vp, C.c_int, _i32p]
+        L.om_pattern.argtypes = [vp, C.c_int, _i32p, _i32p]
+        L.om_lu_ops.argtypes = [vp, C.c_int, _i32p, _i32p, _i32p]
+        L.om_rates_block.argtypes = [vp, C.c_int] + [_f64p] * 7
+        L.om_feval.argtypes = [vp, C.c_int, C.c_int, _f64p, _f64p, _f64p]
+        L.om_jacob.argtypes = [vp, C.c_int, C.c_int, _f64p, _f64p, _f64p]
+        L.om_decomp.argtypes = [vp, C.c_int, C.c_int, _f64p, _f64p]
+        L.om_solve.argtypes = [vp, C.c_int, C.c_int, _f64p, _f64p, _f64p]
+        L.om_rbsolver.argtypes = [vp, C.c_int, C.c_int, C.c_double, _f64p, _f64p, _f64p, _f64p,
+                                  _i32p, _i32p, C.POINTER(_Fail)]
+        L.om_run.argtypes = [vp, C.c_int64, C.c_int, C.c_double] + [_f64p] * 7 + [_f64p, _f64p,
+                                                                               _i32p, _i32p, C.POINTER(_Fail)]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    """RBSPARSE state for one mechanism plus the numeric kernels."""
+
+    def __init__(self, mech):
+        m = mech
+        self.mech = m
+        self.ns, self.nr = m["numb_mech_spc"], m["nrxns"]
+        self.nphotab, self.nhetero = m["nphotab"], m["nhetero"]
+        keep = {}
+
+        def i32(x):
+            a = np.ascontiguousarray(np.asarray(x, dtype=np.int32).reshape(-1)) if len(x) else np.zeros(1, np.int32)
+            keep[id(a)] = a
+            return _p(a, _i32p)
+
+        def f64(x):
+            a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1))
+            keep[id(a)] = a
+            return _p(a, _f64p)
+
+        t = _Tables()
+        t.ns, t.nr, t.mxprd, t.kunits = self.ns, self.nr, m["mxprd"], m["kunits"]
+        t.irr, t.sc = i32(m["irr"]), f64(m["sc"])
+        t.nreact, t.nprdct, t.ktype = i32(m["nreact"]), i32(m["nprdct"]), i32(m["ktype"])
+        t.irxbits, t.iorder, t.rtdat = i32(m["irxbits"]), i32(m["iorder"]), f64(m["rtdat"])
+        t.nfalloff, t.irrfall, t.rfdat = m["nfalloff"], i32(m["irrfall"]), f64(m["rfdat"])
+        t.nmphot, t.nphotab, t.iph = m["nmphot"], m["nphotab"], i32(m["iph"])
+        t.mhetero, t.nhetero, t.ihetero = m["mhetero"], m["nhetero"], i32(m["ihetero"])
+        for key in ("nrxwm", "nrxww", "nrxwo2", "nrxwn2", "nrxwch4", "nrxwh2"):
+            setattr(t, key, i32(m[key]))
+        t.nwm, t.nww, t.nwo2 = len(m["nrxwm"]), len(m["nrxww"]), len(m["nrxwo2"])
+        t.nwn2, t.nwch4, t.nwh2 = len(m["nrxwn2"]), len(m["nrxwch4"]), len(m["nrxwh2"])
+        t.atm_air, t.atm_o2, t.atm_n2 = m["atm_air"], m["atm_o2"], m["atm_n2"]
+        t.atm_ch4, t.atm_h2 = m["atm_ch4"], m["atm_h2"]
+        self._keep, self._t = keep, t
+        err = C.create_string_buffer(256)
+        self._ctx = lib().om_create(C.byref(t), err, 256)
+        if not self._ctx:
+            raise ValueError(err.value.decode())
+
+    def __del__(self):
+        if getattr(self, "_ctx", None) and _lib is not None:
+            _lib.om_destroy(self._ctx)
+            self._ctx = None
+
+    # ---- symbolic ----
+    @property
+    def ischan(self):
+        return lib().om_ischan(self._ctx)
+
+    def iarray(self, ncsp):
+        return lib().om_iarray(self._ctx, ncsp)
+
+    def inew2old(self):
+        out = np.zeros(self.ns, np.int32)
+        lib().om_inew2old(self._ctx, _p(out, _i32p))
+        return out
+
+    def nkuserat(self, ncsp):
+        out = np.zeros(lib().om_nuserat(self._ctx, ncsp), np.int32)
+        lib().om_nkuserat(self._ctx, ncsp, _p(out, _i32p))
+        return out
+
+    def pattern(self, ncsp):
+        n = self.iarray(ncsp)
+        rows, cols = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        lib().om_pattern(self._ctx, ncsp, _p(rows, _i32p), _p(cols, _i32p))
+        return rows, cols
+
+    def lu_ops(self, ncsp):
+        n = lib().om_nlu_ops(self._ctx, ncsp)
+        a, b, c = (np.zeros(n, np.int32) for _ in range(3))
+        lib().om_lu_ops(self._ctx, ncsp, _p(a, _i32p), _p(b, _i32p), _p(c, _i32p))
+        return a, b, c
+
+    # ---- numeric (block = all given cells) ----
+    def rates(self, temp, pres_atm, h2o_ppm, rj, het=None, seawater=None):
+        temp = np.ascontiguousarray(temp, np.float64)
+        nc = temp.size
+        pres_atm = np.ascontiguousarray(pres_atm, np.float64)
+        h2o_ppm = np.ascontiguousarray(h2o_ppm, np.float64)
+        rj = np.ascontiguousarray(rj, np.float64).reshape(self.nphotab, nc)
+        het = np.zeros((max(self.nhetero, 1), nc)) if het is None else np.ascontiguousarray(het, np.float64)
+        seawater = np.zeros(nc) if seawater is None else np.ascontiguousarray(seawater, np.float64)
+        rki = np.zeros((self.nr, nc))
+        sun = lib().om_rates_block(self._ctx, nc, _p(temp, _f64p), _p(pres_atm, _f64p), _p(h2o_ppm, _f64p),
+                                   _p(rj, _f64p), _p(het, _f64p), _p(seawater, _f64p), _p(rki, _f64p))
+        return rki, bool(sun)
+
+    def feval(self, ncsp, rki, y_new):
+        rki = np.ascontiguousarray(rki, np.float64)
+        y_new = np.ascontiguousarray(y_new, np.float64)
+        nc = y_new.shape[1]
+        ydot = np.zeros_like(y_new)
+        lib().om_feval(self._ctx, ncsp, nc, _p(rki, _f64p), _p(y_new, _f64p), _p(ydot, _f64p))
+        return ydot
+
+    def jacob(self, ncsp, rki, y_new):
+        rki = np.ascontiguousarray(rki, np.float64)
+        y_new = np.ascontiguousarray(y_new, np.float64)
+        nc = y_new.shape[1]
+        cc2 = np.zeros((self.iarray(ncsp) + 1, nc))
+        lib().om_jacob(self._ctx, ncsp, nc, _p(rki, _f64p), _p(y_new, _f64p), _p(cc2, _f64p))
+        return cc2
+
+    def decomp(self, ncsp, cc2):
+        cc2 = np.array(cc2, np.float64, copy=True, order="C")
+        nc = cc2.shape[1]
+        vdiag = np.zeros((self.ischan, nc))
+        lib().om_decomp(self._ctx, ncsp, nc, _p(cc2, _f64p), _p(vdiag, _f64p))
+        return cc2, vdiag
+
+    def solve(self, ncsp, cc2, vdiag, rhs):
+        cc2 = np.ascontiguousarray(cc2, np.float64)
+        vdiag = np.ascontiguousarray(vdiag, np.float64)
+        rhs = np.array(rhs, np.float64, copy=True, order="C")
+        lib().om_solve(self._ctx, ncsp, rhs.shape[1], _p(cc2, _f64p), _p(vdiag, _f64p), _p(rhs, _f64p))
+        return rhs
+
+    def rbsolver(self, ncsp, chemstep, rki, y_new, rtol_new, atol_new):
+        rki = np.ascontiguousarray(rki, np.float64)
+        y = np.array(y_new, np.float64, copy=True, order="C")
+        rt = np.ascontiguousarray(rtol_new, np.float64)
+        at = np.ascontiguousarray(atol_new, np.float64)
+        na, nj, f = C.c_int32(), C.c_int32(), _Fail()
+        rc = lib().om_rbsolver(self._ctx, ncsp, y.shape[1], chemstep, _p(rki, _f64p), _p(y, _f64p),
+                               _p(rt, _f64p), _p(at, _f64p), C.byref(na), C.byref(nj), C.byref(f))
+        return rc, y, na.value, nj.value
+
+    def run(self, chemstep, temp, pres_atm, h2o_ppm, rj, y, rtol, atol, het=None, seawater=None, blksize=1):
+        """Whole-domain driver; y is [ns, ncells] in mechanism order (copied)."""
+        temp = np.ascontiguousarray(temp, np.float64)
+        nc = temp.size
+        args = [np.ascontiguousarray(a, np.float64) for a in (pres_atm, h2o_ppm, rj)]
+        het = np.zeros((max(self.nhetero, 1), nc)) if het is None else np.ascontiguousarray(het, np.float64)
+        sw = np.zeros(nc) if seawater is None else np.ascontiguousarray(seawater, np.float64)
+        yy = np.array(y, np.float64, copy=True, order="C")
+        rt = np.ascontiguousarray(np.broadcast_to(rtol, (self.ns,)), np.float64)
+        at = np.ascontiguousarray(np.broadcast_to(atol, (self.ns,)), np.float64)
+        na, nj = np.zeros(nc, np.int32), np.zeros(nc, np.int32)
+        f = _Fail()
+        rc = lib().om_run(self._ctx, nc, blksize, chemstep, _p(temp, _f64p), _p(args[0], _f64p),
+                          _p(args[1], _f64p), _p(args[2], _f64p), _p(het, _f64p), _p(sw, _f64p),
+                          _p(yy, _f64p), _p(rt, _f64p), _p(at, _f64p), _p(na, _i32p), _p(nj, _i32p),
+                          C.byref(f))
+        return {"rc": rc, "y": yy, "nacc": na, "nrej": nj,
+                "fail": {"cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}}

@@ -1,0 +1,752 @@
+/*
+ * CPU oracle for CMAQ's Rosenbrock ROS3 gas-chemistry path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see ros3_oracle.h).  A plain-C restatement of
+ *   CCTM/src/gas/ros3/rbsparse.F  (symbolic analysis)      -> om_create
+ *   CCTM/src/gas/ros3/rbfeval.F   (f(y))                   -> om_feval
+ *   CCTM/src/gas/ros3/rbjacob.F   (Jacobian)               -> om_jacob
+ *   CCTM/src/gas/ros3/rbdecomp.F  (sparse LU)              -> om_decomp
+ *   CCTM/src/gas/ros3/rbsolve.F   (triangular solves)      -> om_solve
+ *   CCTM/src/gas/ros3/rbsolver.F  (ROS3 + step control)    -> om_rbsolver
+ *   CCTM/src/gas/ros3/rbdriver.F:440-595 (block driver)    -> om_run
+ *   MECHS/<mech>/RXNS_FUNC_MODULE.F90 CALC_RCONST, as written by
+ *   UTIL/chemmech/src/wrt_reactions_module.f:580-800       -> om_rates_block
+ * Arithmetic follows the Fortran expression order term by term; compile
+ * with -ffp-contract=off (the reference's gcc flags, config_cmaq.csh:164,
+ * do not enable FMA contraction on x86-64).
+ *
+ * Arrays here keep the Fortran 1-based indexing (slot 0 unused) so that each
+ * loop can be compared line by line with the reference.
+ */
+#include "ros3_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MXRCT 3
+
+struct om_ctx {
+    om_tables t;
+    int ns, nr, mxprd, ischan;
+    int *inew2old, *iold2new;          /* [ns+1] */
+    int *irm2;                         /* [(nr+1)*(mxprd+4)] */
+    int nuserat[3];
+    int *nkuserat[3];                  /* [nr+1] */
+    int iarray[3];
+    int *jarraypt[3];                  /* [(ns+1)*(ns+1)], (row, col) */
+    int *kzero[3];                     /* [iarray+1] column of slot */
+    int *kzilch[3], *mzilch[3];        /* [ns+1] */
+    /* decomposition: loop-1 ops of column j are ops[opbeg[j] .. opbeg[j+1]) */
+    int nops[3];
+    int *opbeg[3], *op_ij[3], *op_ik[3], *op_kj[3];
+    /* loop-2 (L scaling): slots lsc[lbeg[j] .. lbeg[j+1]) */
+    int *lbeg[3], *lsc[3];
+    /* PD lists */
+    int mxrr, mxrp;
+    int *nderivl[3], *nderivp[3];
+    int *jarrl[3], *jlial[3], *jarrp[3], *jpial[3], *icoeff[3];
+    /* rate evaluation: falloff row and het table per reaction */
+    int *fall_idx;                     /* [nr+1] 1-based RFDAT column or 0 */
+    int *het_idx;                      /* [nr+1] */
+};
+
+#define IRR(c, r, s)  ((c)->t.irr[((r) - 1) * ((c)->mxprd + 3) + (s) - 1])
+#define SC(c, r, p)   ((c)->t.sc[((r) - 1) * (c)->mxprd + (p) - 1])
+#define RTDAT(c, k, r) ((c)->t.rtdat[((r) - 1) * 3 + (k) - 1])
+#define RFDAT(c, k, f) ((c)->t.rfdat[((f) - 1) * 5 + (k) - 1])
+#define IRM2(c, r, s) ((c)->irm2[(r) * ((c)->mxprd + 4) + (s)])
+#define JAP(c, n, i, j) ((c)->jarraypt[n][(i) * ((c)->ns + 1) + (j)])
+
+static void *xcalloc(size_t n, size_t s) { void *p = calloc(n ? n : 1, s); if (!p) abort(); return p; }
+
+void om_destroy(om_ctx *c)
+{
+    if (!c) return;
+    free(c->inew2old); free(c->iold2new); free(c->irm2);
+    for (int n = 1; n <= 2; ++n) {
+        free(c->nkuserat[n]); free(c->jarraypt[n]); free(c->kzero[n]);
+        free(c->kzilch[n]); free(c->mzilch[n]); free(c->opbeg[n]);
+        free(c->op_ij[n]); free(c->op_ik[n]); free(c->op_kj[n]);
+        free(c->lbeg[n]); free(c->lsc[n]);
+        free(c->nderivl[n]); free(c->nderivp[n]); free(c->jarrl[n]);
+        free(c->jlial[n]); free(c->jarrp[n]); free(c->jpial[n]); free(c->icoeff[n]);
+    }
+    free(c->fall_idx); free(c->het_idx);
+    free(c);
+}
+
+om_ctx *om_create(const om_tables *t, char *err, int errlen)
+{
+    om_ctx *c = xcalloc(1, sizeof *c);
+    c->t = *t;
+    const int ns = c->ns = t->ns, nr = c->nr = t->nr, mxprd = c->mxprd = t->mxprd;
+    const int n1 = ns + 1;
+    c->mxrr = 3 * MXRCT;             /* rbinit.F:101 */
+    c->mxrp = 3 * mxprd;             /* rbinit.F:102 */
+
+    /* ---- rate-table lookups and unsupported features ---- */
+    c->fall_idx = xcalloc(nr + 1, sizeof(int));
+    c->het_idx = xcalloc(nr + 1, sizeof(int));
+    for (int f = t->nfalloff; f >= 1; --f) c->fall_idx[t->irrfall[f - 1]] = f;   /* first match */
+    for (int h = t->mhetero; h >= 1; --h) c->het_idx[t->ihetero[(h - 1) * 2]] = t->ihetero[(h - 1) * 2 + 1];
+    for (int r = 1; r <= nr; ++r) {
+        int k = t->ktype[r - 1];
+        if (k == 7 || k == 11 || k == 13 || k < -1 || k > 13) {
+            snprintf(err, errlen, "reaction %d: rate type %d not supported", r, k);
+            om_destroy(c); return NULL;
+        }
+        if ((k >= 8 && k <= 10) || k == 12) {
+            if (!c->fall_idx[r]) { snprintf(err, errlen, "reaction %d: no RFDAT row", r); om_destroy(c); return NULL; }
+        }
+        if (k == 12 && t->iorder[r - 1] != 1) {
+            snprintf(err, errlen, "reaction %d: halogen rate of order %d not supported", r, t->iorder[r - 1]);
+            om_destroy(c); return NULL;
+        }
+        if (k == -1 && !c->het_idx[r]) { snprintf(err, errlen, "reaction %d: no IHETERO row", r); om_destroy(c); return NULL; }
+        if (t->nreact[r - 1] > MXRCT) { snprintf(err, errlen, "reaction %d: >3 reactants", r); om_destroy(c); return NULL; }
+    }
+    for (int n = 0; n < t->nmphot; ++n) {
+        if (t->iph[n * 3 + 2] == 0) {
+            snprintf(err, errlen, "IPH(%d,3)=0 (photolysis relative to a reaction) not supported", n + 1);
+            om_destroy(c); return NULL;
+        }
+    }
+
+    /* ---- rbsparse.F:185-202: PD terms per species ---- */
+    int *isaporl = xcalloc(n1, sizeof(int));
+    char *isparder = xcalloc((size_t)n1 * n1, 1);      /* (iporr, ireact) */
+    for (int nrx = 1; nrx <= nr; ++nrx)
+        for (int nrr = 1; nrr <= 3; ++nrr) {
+            int ireact = IRR(c, nrx, nrr);
+            if (ireact != 0)
+                for (int nrpp = 1; nrpp <= 3 + mxprd; ++nrpp) {
+                    int iporr = IRR(c, nrx, nrpp);
+                    if (iporr != 0) isparder[iporr * n1 + ireact] = 1;
+                }
+        }
+    for (int ireact = 1; ireact <= ns; ++ireact)
+        for (int iporr = 1; iporr <= ns; ++iporr)
+            if (isparder[iporr * n1 + ireact]) isaporl[iporr]++;
+
+    /* ---- rbsparse.F:209-252: species order ---- */
+    c->inew2old = xcalloc(n1, sizeof(int));
+    c->iold2new = xcalloc(n1, sizeof(int));
+    int ischan = 0, nochang = ns;
+    for (int jold = 1; jold <= ns; ++jold) {
+        if (isaporl[jold] > 0) {
+            ++ischan; c->inew2old[ischan] = jold; c->iold2new[jold] = ischan;
+        } else {
+            c->inew2old[nochang] = jold; c->iold2new[jold] = nochang; --nochang;
+        }
+    }
+    for (int jnew = 1; jnew <= ischan; ++jnew) {
+        int jold = c->inew2old[jnew], minvalu = isaporl[jold], iminold = jold, iminnew = jnew;
+        for (int inew = jnew + 1; inew <= ischan; ++inew) {
+            int iold = c->inew2old[inew];
+            if (isaporl[iold] < minvalu) { minvalu = isaporl[iold]; iminold = iold; iminnew = inew; }
+        }
+        c->inew2old[iminnew] = jold; c->inew2old[jnew] = iminold;
+        c->iold2new[jold] = iminnew; c->iold2new[iminold] = jnew;
+    }
+    c->ischan = ischan;
+    free(isaporl); free(isparder);
+
+    /* ---- rbsparse.F:259-281: IRM2 and active reaction lists ---- */
+    c->irm2 = xcalloc((size_t)(nr + 1) * (mxprd + 4), sizeof(int));
+    for (int n = 1; n <= 2; ++n) c->nkuserat[n] = xcalloc(nr + 1, sizeof(int));
+    for (int nrx = 1; nrx <= nr; ++nrx) {
+        for (int nrr = 1; nrr <= t->nreact[nrx - 1]; ++nrr) IRM2(c, nrx, nrr) = c->iold2new[IRR(c, nrx, nrr)];
+        for (int np = 1; np <= t->nprdct[nrx - 1]; ++np) IRM2(c, nrx, np + 3) = c->iold2new[IRR(c, nrx, np + 3)];
+        if (t->nreact[nrx - 1] > 0) {
+            c->nkuserat[1][++c->nuserat[1]] = nrx;
+            if (!(t->irxbits[nrx - 1] & 2)) c->nkuserat[2][++c->nuserat[2]] = nrx;
+        }
+    }
+
+    /* ---- rbsparse.F:360-655, once for day (1) and night (2) ---- */
+    char *lzero = xcalloc((size_t)n1 * n1, 1);           /* (row, col) */
+#define LZ(i, j) lzero[(i) * n1 + (j)]
+    for (int ncsp = 1; ncsp <= 2; ++ncsp) {
+        memset(lzero, 0, (size_t)n1 * n1);
+        for (int i = 1; i <= ns; ++i) LZ(i, i) = 1;
+        for (int nrx = 1; nrx <= c->nuserat[ncsp]; ++nrx) {
+            int nk = c->nkuserat[ncsp][nrx];
+            for (int nrr = 1; nrr <= t->nreact[nk - 1]; ++nrr) {
+                int ire = IRM2(c, nk, nrr);
+                for (int jal = 1; jal <= t->nreact[nk - 1]; ++jal) LZ(IRM2(c, nk, jal), ire) = 1;
+                for (int iap = 1; iap <= t->nprdct[nk - 1]; ++iap) LZ(IRM2(c, nk, 3 + iap), ire) = 1;
+            }
+        }
+        /* symbolic decomposition (:423-455) */
+        int cap = 16, nop = 0, nls = 0, lcap = 16;
+        int *oi = xcalloc(cap, sizeof(int)), *ok = xcalloc(cap, sizeof(int)), *oj = xcalloc(cap, sizeof(int));
+        int *opcol = xcalloc(ischan + 2, sizeof(int));
+        int *li = xcalloc(lcap, sizeof(int)), *lj = xcalloc(lcap, sizeof(int));
+        int *lcol = xcalloc(ischan + 2, sizeof(int));
+        for (int j = 1; j <= ischan; ++j) {
+            int j1 = j - 1;
+            opcol[j] = nop;
+            for (int i = 2; i <= ischan; ++i) {
+                int i1 = j1;
+                if (i <= j1) i1 = i - 1;
+                for (int k = 1; k <= i1; ++k) {
+                    if (LZ(i, k) == 1 && LZ(k, j) == 1) {
+                        if (nop == cap) {
+                            cap *= 2;
+                            oi = realloc(oi, cap * sizeof(int)); ok = realloc(ok, cap * sizeof(int)); oj = realloc(oj, cap * sizeof(int));
+                        }
+                        oi[nop] = i; ok[nop] = k; oj[nop] = j; ++nop;
+                        LZ(i, j) = 1;
+                    }
+                }
+            }
+            lcol[j] = nls;
+            for (int i = j + 1; i <= ischan; ++i) {
+                if (LZ(i, j) == 1) {
+                    if (nls == lcap) { lcap *= 2; li = realloc(li, lcap * sizeof(int)); lj = realloc(lj, lcap * sizeof(int)); }
+                    li[nls] = i; lj[nls] = j; ++nls;
+                }
+            }
+        }
+        opcol[ischan + 1] = nop; lcol[ischan + 1] = nls;
+
+        /* storage layout (:461-497) */
+        c->jarraypt[ncsp] = xcalloc((size_t)n1 * n1, sizeof(int));
+        c->kzilch[ncsp] = xcalloc(n1, sizeof(int));
+        c->mzilch[ncsp] = xcalloc(n1, sizeof(int));
+        int nnz = 0;
+        for (int i = 1; i <= ischan; ++i) for (int j = 1; j <= ischan; ++j) nnz += LZ(i, j);
+        c->kzero[ncsp] = xcalloc(nnz + 1, sizeof(int));
+        int ia = 0;
+        for (int i = 2; i <= ischan; ++i)
+            for (int j = 1; j <= i - 1; ++j)
+                if (LZ(i, j) == 1) { c->kzilch[ncsp][i]++; ++ia; c->kzero[ncsp][ia] = j; JAP(c, ncsp, i, j) = ia; }
+        for (int i = ischan - 1; i >= 1; --i)
+            for (int j = i + 1; j <= ischan; ++j)
+                if (LZ(i, j) == 1) { c->mzilch[ncsp][i]++; ++ia; c->kzero[ncsp][ia] = j; JAP(c, ncsp, i, j) = ia; }
+        for (int i = 1; i <= ischan; ++i) { ++ia; JAP(c, ncsp, i, i) = ia; c->kzero[ncsp][ia] = i; }
+        c->iarray[ncsp] = ia;
+
+        /* decomposition op lists in storage pointers (:503-554) */
+        c->nops[ncsp] = nop;
+        c->opbeg[ncsp] = xcalloc(ischan + 2, sizeof(int));
+        c->op_ij[ncsp] = xcalloc(nop + 1, sizeof(int));
+        c->op_ik[ncsp] = xcalloc(nop + 1, sizeof(int));
+        c->op_kj[ncsp] = xcalloc(nop + 1, sizeof(int));
+        for (int j = 1; j <= ischan + 1; ++j) c->opbeg[ncsp][j] = opcol[j];
+        for (int q = 0; q < nop; ++q) {
+            c->op_ij[ncsp][q] = JAP(c, ncsp, oi[q], oj[q]);
+            c->op_ik[ncsp][q] = JAP(c, ncsp, oi[q], ok[q]);
+            c->op_kj[ncsp][q] = JAP(c, ncsp, ok[q], oj[q]);
+        }
+        c->lbeg[ncsp] = xcalloc(ischan + 2, sizeof(int));
+        c->lsc[ncsp] = xcalloc(nls + 1, sizeof(int));
+        for (int j = 1; j <= ischan + 1; ++j) c->lbeg[ncsp][j] = lcol[j];
+        for (int q = 0; q < nls; ++q) c->lsc[ncsp][q] = JAP(c, ncsp, li[q], lj[q]);
+        free(oi); free(ok); free(oj); free(opcol); free(li); free(lj); free(lcol);
+
+        /* PD pointers (:626-655) */
+        c->nderivl[ncsp] = xcalloc(nr + 1, sizeof(int));
+        c->nderivp[ncsp] = xcalloc(nr + 1, sizeof(int));
+        c->jarrl[ncsp] = xcalloc((size_t)(nr + 1) * (c->mxrr + 1), sizeof(int));
+        c->jlial[ncsp] = xcalloc((size_t)(nr + 1) * (c->mxrr + 1), sizeof(int));
+        c->jarrp[ncsp] = xcalloc((size_t)(nr + 1) * (c->mxrp + 1), sizeof(int));
+        c->jpial[ncsp] = xcalloc((size_t)(nr + 1) * (c->mxrp + 1), sizeof(int));
+        c->icoeff[ncsp] = xcalloc((size_t)(nr + 1) * (c->mxrp + 1), sizeof(int));
+        for (int nrx = 1; nrx <= c->nuserat[ncsp]; ++nrx) {
+            int nk = c->nkuserat[ncsp][nrx];
+            for (int ial = 1; ial <= t->nreact[nk - 1]; ++ial) {
+                int ir = IRM2(c, nk, ial);
+                for (int jal = 1; jal <= t->nreact[nk - 1]; ++jal) {
+                    int jr = IRM2(c, nk, jal);
+                    int nl = ++c->nderivl[ncsp][nk];
+                    c->jarrl[ncsp][nk * (c->mxrr + 1) + nl] = JAP(c, ncsp, jr, ir);
+                    c->jlial[ncsp][nk * (c->mxrr + 1) + nl] = ial;
+                }
+                for (int iap = 1; iap <= t->nprdct[nk - 1]; ++iap) {
+                    int jp = IRM2(c, nk, iap + 3);
+                    int npr = ++c->nderivp[ncsp][nk];
+                    c->jarrp[ncsp][nk * (c->mxrp + 1) + npr] = JAP(c, ncsp, jp, ir);
+                    c->jpial[ncsp][nk * (c->mxrp + 1) + npr] = ial;
+                    c->icoeff[ncsp][nk * (c->mxrp + 1) + npr] = (fabs(SC(c, nk, iap) - 1.0) > 1.0e-06) ? iap : 0;
+                }
+            }
+        }
+    }
+#undef LZ
+    free(lzero);
+    return c;
+}
+
+int om_ischan(const om_ctx *c) { return c->ischan; }
+int om_iarray(const om_ctx *c, int ncsp) { return c->iarray[ncsp]; }
+void om_inew2old(const om_ctx *c, int32_t *out) { for (int i = 1; i <= c->ns; ++i) out[i - 1] = c->inew2old[i]; }
+int om_nuserat(const om_ctx *c, int ncsp) { return c->nuserat[ncsp]; }
+void om_nkuserat(const om_ctx *c, int ncsp, int32_t *out) { for (int i = 1; i <= c->nuserat[ncsp]; ++i) out[i - 1] = c->nkuserat[ncsp][i]; }
+int om_nlu_ops(const om_ctx *c, int ncsp) { return c->nops[ncsp]; }
+void om_lu_ops(const om_ctx *c, int ncsp, int32_t *ij, int32_t *ik, int32_t *kj)
+{
+    for (int q = 0; q < c->nops[ncsp]; ++q) { ij[q] = c->op_ij[ncsp][q]; ik[q] = c->op_ik[ncsp][q]; kj[q] = c->op_kj[ncsp][q]; }
+}
+void om_pattern(const om_ctx *c, int ncsp, int32_t *rows, int32_t *cols)
+{
+    for (int i = 1; i <= c->ischan; ++i)
+        for (int j = 1; j <= c->ischan; ++j) {
+            int s = JAP(c, ncsp, i, j);
+            if (s) { rows[s - 1] = i; cols[s - 1] = j; }
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* CALC_RCONST (RXNS_FUNC_MODULE.F90:204-1116 for cb6r5_ae7_aq)        */
+/* ------------------------------------------------------------------ */
+static double power_t02(double t300, double a0, double b0) { return a0 * pow(t300, b0); }
+static double arr_t03(double inv_t, double a0, double b0) { return a0 * exp(b0 * inv_t); }
+static double arr_t04(double inv_t, double t300, double a0, double b0, double c0)
+{ return a0 * exp(b0 * inv_t) * pow(t300, c0); }
+static double fall_t08(double inv_t, double cair, double a0, double c0, double a2, double c2, double a3, double c3)
+{
+    double k0 = a0 * exp(c0 * inv_t), k2 = a2 * exp(c2 * inv_t), k3 = a3 * exp(c3 * inv_t);
+    k3 = k3 * cair;
+    return k0 + k3 / (1.0 + k3 / k2);
+}
+static double fall_t09(double inv_t, double cair, double a1, double c1, double a2, double c2)
+{
+    double k1 = a1 * exp(c1 * inv_t), k2 = a2 * exp(c2 * inv_t);
+    return k1 + k2 * cair;
+}
+static double fall_t10(double inv_t, double t300, double cair, double a0, double b0, double c0,
+                       double a1, double b1, double c1, double ce, double cf)
+{
+    double k0 = a0 * cair * exp(b0 * inv_t) * pow(t300, c0);
+    double k1 = a1 * exp(b1 * inv_t) * pow(t300, c1);
+    double g = (1.0 / ce) * log10(k0 / k1);
+    double kend = 1.0 + g * g;
+    kend = 1.0 / kend;
+    return (k0 / (1.0 + k0 / k1)) * pow(cf, kend);
+}
+static double fall_t11(double inv_t, double t300, double cair, double a1, double b1, double c1,
+                       double a2, double b2, double c2, double d1, double d2)
+{
+    double k1 = a1 * exp(c1 * inv_t) * pow(t300, b1);
+    double k2 = a2 * exp(c2 * inv_t) * pow(t300, b2);
+    double k3 = d1 * exp(d2 * inv_t);
+    return k1 + k2 * cair + k3;
+}
+static double halogen_falloff(double p, double a1, double b1, double a2, double b2, double a3)
+{
+    double h = a1 * exp(b1 * p) + a2 * exp(b2 * p);
+    return a3 < h ? a3 : h;
+}
+/* chemmech writes computed constants with FORMAT 1PD12.4 */
+static double d12_4(double v) { char b[64]; snprintf(b, sizeof b, "%.4e", v); return strtod(b, NULL); }
+
+typedef struct { double cfact, cfact_squ, inv_cfact, sfact, rfact, rfact_squ; } om_conv;
+/* WRITE_RATE_CONVERT_BEFORE/AFTER (UTIL/chemmech/src/BASIC_WRITE_ROUTINES.f:1102-1152);
+ * returns 0 when chemmech writes no factor for this order. */
+static int conv_factor(const om_conv *u, int order, double *f)
+{
+    switch (order) {
+    case 0: *f = u->inv_cfact; return 1;
+    case 1: *f = u->sfact; return 1;
+    case 2: *f = u->cfact; return 1;
+    case 3: *f = u->cfact_squ; return 1;
+    case -1: *f = u->rfact; return 1;
+    case -2: *f = u->rfact_squ; return 1;
+    default: return 0;
+    }
+}
+
+int om_rates_block(const om_ctx *c, int nc, const double *temp, const double *pres,
+                   const double *h2o, const double *rj, const double *het,
+                   const double *seawater, double *rki)
+{
+    const om_tables *t = &c->t;
+    const double COEF1 = 7.33981e+15, TI300 = 1.0 / 300.0, SFACT = 60.0;
+    int sunlit = 0;
+    for (int i = 0; i < t->nphotab; ++i)
+        for (int cc = 0; cc < nc; ++cc)
+            if (rj[(size_t)i * nc + cc] > 0.0) sunlit = 1;
+    for (size_t q = 0; q < (size_t)t->nr * nc; ++q) rki[q] = 0.0;
+#define RK(r) rki[(size_t)((r) - 1) * nc + cc]
+    if (sunlit) {
+        for (int cc = 0; cc < nc; ++cc) {
+            for (int n = 0; n < t->nmphot; ++n) {
+                int nxx = t->iph[n * 3], idx = t->iph[n * 3 + 1];
+                if (nxx <= 0) continue;
+                double j = rj[(size_t)(idx - 1) * nc + cc];
+                RK(nxx) = (RTDAT(c, 1, nxx) != 1.0) ? RTDAT(c, 1, nxx) * j : j;
+            }
+            for (int nxx = 1; nxx <= t->nr; ++nxx) {
+                if (t->ktype[nxx - 1] != 12) continue;
+                if (seawater[cc] > 0.001) {
+                    int f = c->fall_idx[nxx];
+                    double h = halogen_falloff(pres[cc], RTDAT(c, 1, nxx), RFDAT(c, 1, f),
+                                               RTDAT(c, 2, nxx), RFDAT(c, 2, f), RTDAT(c, 3, nxx));
+                    RK(nxx) = seawater[cc] * SFACT * h;     /* first order only */
+                } else {
+                    RK(nxx) = 0.0;
+                }
+            }
+        }
+    }
+    for (int cc = 0; cc < nc; ++cc) {
+        double inv_temp = 1.0 / temp[cc];
+        double cair = 1.0e+06 * COEF1 * pres[cc] * inv_temp;
+        om_conv u;
+        u.cfact = 6.0e-05 * cair;
+        u.cfact_squ = 6.0e-11 * cair * cair;
+        double inv_cair = 1.0 / cair;
+        u.inv_cfact = 6.0e+07 * inv_cair;
+        double t300 = temp[cc] * TI300;
+        u.rfact = 1.0e+06 * inv_cair;
+        u.rfact_squ = 1.0e+12 * inv_cair * inv_cair;
+        u.sfact = SFACT;
+        for (int nxx = 1; nxx <= t->nr; ++nxx) {
+            int kt = t->ktype[nxx - 1], ord = t->iorder[nxx - 1];
+            double a = RTDAT(c, 1, nxx), b = RTDAT(c, 2, nxx), cc3 = RTDAT(c, 3, nxx);
+            double f, v;
+            int f_ok;
+            int fi = c->fall_idx[nxx];
+            switch (kt) {
+            case -1: {
+                double h = het[(size_t)(c->het_idx[nxx] - 1) * nc + cc];
+                RK(nxx) = (a != 1.0) ? a * h : h;
+                break;
+            }
+            case 1:
+                RK(nxx) = conv_factor(&u, ord, &f) ? a * f : a;
+                break;
+            case 2:
+                v = power_t02(t300, a, b);
+                RK(nxx) = (t->kunits == 2 && conv_factor(&u, ord, &f)) ? f * v : v;
+                break;
+            case 3:
+                v = arr_t03(inv_temp, a, cc3);
+                RK(nxx) = (t->kunits == 2 && conv_factor(&u, ord, &f)) ? f * v : v;
+                break;
+            case 4:
+                v = arr_t04(inv_temp, t300, a, cc3, b);
+                RK(nxx) = (t->kunits == 2 && conv_factor(&u, ord, &f)) ? f * v : v;
+                break;
+            case 5: {
+                int irx = (int)cc3;
+                if (irx > nxx) break;                      /* never written by chemmech */
+                int idiff = ord - t->iorder[irx - 1];
+                int kti = t->ktype[irx - 1];
+                int falloff = (kti > 7 && kti < 11);
+                double eq = d12_4(1.0 / a) * exp(-b * inv_temp);
+                if (idiff != 0 && (t->kunits == 2 || falloff) && conv_factor(&u, idiff, &f))
+                    RK(nxx) = f * RK(irx) * eq;
+                else
+                    RK(nxx) = RK(irx) * eq;
+                break;
+            }
+            case 6: {
+                int irx = (int)b;
+                if (irx > nxx) break;                      /* never written by chemmech */
+                int idiff = ord - t->iorder[irx - 1];
+                int kti = t->ktype[irx - 1];
+                int falloff = (kti > 7 && kti < 11);
+                f_ok = idiff != 0 && (t->kunits == 2 || falloff) && conv_factor(&u, idiff, &f);
+                if (a != 1.0) RK(nxx) = f_ok ? f * a * RK(irx) : a * RK(irx);
+                else RK(nxx) = f_ok ? f * RK(irx) : RK(irx);
+                break;
+            }
+            case 8:
+                v = fall_t08(inv_temp, cair, a, b, cc3, RFDAT(c, 1, fi), RFDAT(c, 2, fi), RFDAT(c, 3, fi));
+                RK(nxx) = conv_factor(&u, ord, &f) ? f * v : v;
+                break;
+            case 9:
+                if (RFDAT(c, 2, fi) == 0.0 && RFDAT(c, 3, fi) == 0.0)
+                    v = fall_t09(inv_temp, cair, a, b, cc3, RFDAT(c, 1, fi));
+                else
+                    v = fall_t11(inv_temp, t300, cair, a, RFDAT(c, 2, fi), b, cc3, RFDAT(c, 3, fi),
+                                 RFDAT(c, 1, fi), RFDAT(c, 4, fi), RFDAT(c, 5, fi));
+                RK(nxx) = conv_factor(&u, ord, &f) ? f * v : v;
+                break;
+            case 10:
+                v = fall_t10(inv_temp, t300, cair, a, cc3, b, RFDAT(c, 1, fi), RFDAT(c, 3, fi),
+                             RFDAT(c, 2, fi), RFDAT(c, 5, fi), RFDAT(c, 4, fi));
+                RK(nxx) = conv_factor(&u, ord, &f) ? f * v : v;
+                break;
+            default:
+                break;                                     /* 0 and 12: sunlit block */
+            }
+        }
+    }
+    /* third-body and fixed-species multipliers (:1065-1115) */
+    struct { int n; const int32_t *l; int h2o; double v; } mult[6] = {
+        {t->nwm, t->nrxwm, 0, t->atm_air}, {t->nwo2, t->nrxwo2, 0, t->atm_o2},
+        {t->nwn2, t->nrxwn2, 0, t->atm_n2}, {t->nww, t->nrxww, 1, 0.0},
+        {t->nwh2, t->nrxwh2, 0, t->atm_h2}, {t->nwch4, t->nrxwch4, 0, t->atm_ch4}};
+    for (int m = 0; m < 6; ++m)
+        for (int q = 0; q < mult[m].n; ++q) {
+            int r = mult[m].l[q];
+            for (int cc = 0; cc < nc; ++cc) RK(r) = RK(r) * (mult[m].h2o ? h2o[cc] : mult[m].v);
+        }
+#undef RK
+    return sunlit;
+}
+
+/* ------------------------------------------------------------------ */
+/* Numerical kernels on one block                                     */
+/* ------------------------------------------------------------------ */
+#define A2(a, i, cc) (a)[(size_t)(i) * nc + (cc)]
+
+void om_feval(const om_ctx *c, int ncsp, int nc, const double *rki, const double *y, double *ydot)
+{
+    const om_tables *t = &c->t;
+    /* y / ydot are indexed by new species id - 1 */
+    for (int isp = 1; isp <= c->ns; ++isp)
+        for (int cc = 0; cc < nc; ++cc) A2(ydot, isp - 1, cc) = 0.0;
+    for (int nrx = 1; nrx <= c->nuserat[ncsp]; ++nrx) {
+        int nrk = c->nkuserat[ncsp][nrx], nre = t->nreact[nrk - 1];
+        for (int cc = 0; cc < nc; ++cc) {
+            double rx = A2(rki, nrk - 1, cc);
+            if (nre >= 1) rx = rx * A2(y, IRM2(c, nrk, 1) - 1, cc);
+            if (nre >= 2) rx = rx * A2(y, IRM2(c, nrk, 2) - 1, cc);
+            if (nre >= 3) rx = rx * A2(y, IRM2(c, nrk, 3) - 1, cc);
+            for (int nrr = 1; nrr <= nre; ++nrr) {
+                int s = IRM2(c, nrk, nrr) - 1;
+                A2(ydot, s, cc) = A2(ydot, s, cc) - rx;
+            }
+            for (int np = 1; np <= t->nprdct[nrk - 1]; ++np) {
+                int s = IRM2(c, nrk, np + 3) - 1;
+                A2(ydot, s, cc) = A2(ydot, s, cc) + SC(c, nrk, np) * rx;
+            }
+        }
+    }
+}
+
+void om_jacob(const om_ctx *c, int ncsp, int nc, const double *rki, const double *y, double *cc2)
+{
+    const om_tables *t = &c->t;
+    for (int iar = 0; iar <= c->iarray[ncsp]; ++iar)
+        for (int cc = 0; cc < nc; ++cc) A2(cc2, iar, cc) = 0.0;
+    double ex[4];
+    for (int nrx = 1; nrx <= c->nuserat[ncsp]; ++nrx) {
+        int nrk = c->nkuserat[ncsp][nrx], nre = t->nreact[nrk - 1];
+        for (int cc = 0; cc < nc; ++cc) {
+            double rk = A2(rki, nrk - 1, cc);
+            if (nre == 1) {
+                ex[1] = rk;
+            } else if (nre == 2) {
+                int jr1 = IRM2(c, nrk, 1) - 1, jr2 = IRM2(c, nrk, 2) - 1;
+                ex[1] = rk * A2(y, jr2, cc);
+                ex[2] = rk * A2(y, jr1, cc);
+            } else if (nre == 3) {
+                int jr1 = IRM2(c, nrk, 1) - 1, jr2 = IRM2(c, nrk, 2) - 1, jr3 = IRM2(c, nrk, 3) - 1;
+                double cr2 = rk * A2(y, jr2, cc);
+                ex[1] = cr2 * A2(y, jr3, cc);
+                ex[2] = rk * A2(y, jr1, cc) * A2(y, jr3, cc);
+                ex[3] = cr2 * A2(y, jr1, cc);
+            }
+            for (int nl = 1; nl <= c->nderivl[ncsp][nrk]; ++nl) {
+                int iarp = c->jarrl[ncsp][nrk * (c->mxrr + 1) + nl];
+                int ialp = c->jlial[ncsp][nrk * (c->mxrr + 1) + nl];
+                A2(cc2, iarp, cc) = A2(cc2, iarp, cc) - ex[ialp];
+            }
+            for (int np = 1; np <= c->nderivp[ncsp][nrk]; ++np) {
+                int q = nrk * (c->mxrp + 1) + np;
+                int iarp = c->jarrp[ncsp][q], ialp = c->jpial[ncsp][q], isc = c->icoeff[ncsp][q];
+                if (isc == 0) A2(cc2, iarp, cc) = A2(cc2, iarp, cc) + ex[ialp];
+                else A2(cc2, iarp, cc) = A2(cc2, iarp, cc) + SC(c, nrk, isc) * ex[ialp];
+            }
+        }
+    }
+}
+
+void om_decomp(const om_ctx *c, int ncsp, int nc, double *cc2, double *vdiag)
+{
+    for (int j = 1; j <= c->ischan; ++j) {
+        for (int q = c->opbeg[ncsp][j]; q < c->opbeg[ncsp][j + 1]; ++q) {
+            int ij = c->op_ij[ncsp][q], ik = c->op_ik[ncsp][q], kj = c->op_kj[ncsp][q];
+            for (int cc = 0; cc < nc; ++cc)
+                A2(cc2, ij, cc) = A2(cc2, ij, cc) - A2(cc2, ik, cc) * A2(cc2, kj, cc);
+        }
+        int iar = JAP(c, ncsp, j, j);
+        for (int cc = 0; cc < nc; ++cc) A2(vdiag, j - 1, cc) = 1.0 / A2(cc2, iar, cc);
+        for (int q = c->lbeg[ncsp][j]; q < c->lbeg[ncsp][j + 1]; ++q) {
+            int ija = c->lsc[ncsp][q];
+            for (int cc = 0; cc < nc; ++cc) A2(cc2, ija, cc) = A2(cc2, ija, cc) * A2(vdiag, j - 1, cc);
+        }
+    }
+}
+
+void om_solve(const om_ctx *c, int ncsp, int nc, const double *cc2, const double *vdiag, double *rhs)
+{
+    const int *kz = c->kzero[ncsp];
+    int ij = 1;
+    for (int i = 1; i <= c->ischan; ++i)
+        for (int q = 0; q < c->kzilch[ncsp][i]; ++q, ++ij)
+            for (int cc = 0; cc < nc; ++cc)
+                A2(rhs, i - 1, cc) = A2(rhs, i - 1, cc) - A2(cc2, ij, cc) * A2(rhs, kz[ij] - 1, cc);
+    for (int i = c->ischan; i >= 1; --i) {
+        for (int q = 0; q < c->mzilch[ncsp][i]; ++q, ++ij)
+            for (int cc = 0; cc < nc; ++cc)
+                A2(rhs, i - 1, cc) = A2(rhs, i - 1, cc) - A2(cc2, ij, cc) * A2(rhs, kz[ij] - 1, cc);
+        for (int cc = 0; cc < nc; ++cc) A2(rhs, i - 1, cc) = A2(rhs, i - 1, cc) * A2(vdiag, i - 1, cc);
+    }
+}
+
+/* ROS3 constants (rbsolver.F:85-113) */
+static const double GAM = 0.43586652150845899941601945119356e+00;
+static const double C21 = -0.10156171083877702091975600115545e+01;
+static const double C31 = 0.40759956452537699824805835358067e+01;
+static const double C32 = 0.92076794298330791242156818474003e+01;
+static const double B1 = 0.10000000000000000000000000000000e+01;
+static const double B2 = 0.61697947043828245592553615689730e+01;
+static const double B3 = -0.42772256543218573326238373806514e+00;
+static const double D1 = 0.50000000000000000000000000000000e+00;
+static const double D2 = -0.29079558716805469821718236208017e+01;
+static const double D3 = 0.22354069897811569627360909276199e+00;
+#define GROW (1.0 / 3.00)
+#define DTSTART 0.5e-01
+#define DTMIN 1.0e-08
+#define DTMAX 1.0e+01
+#define UROUND 1.0e-18
+#define FACMAX 1.0e+01
+#define FACMIN 1.0e-01
+#define FACONE 1.0
+#define CONMIN 1.0e-30
+#define OM_MAX_ATTEMPTS 10000000L
+
+int om_rbsolver(const om_ctx *c, int ncsp, int nc, double chemstep, const double *rki,
+                double *y, const double *rtol, const double *atol,
+                int32_t *nacc, int32_t *nrej, om_fail *fail)
+{
+    const int ns = c->ns, ischan = c->ischan;
+    const double RGAM = 1.0 / GAM;
+    const double rnspec = (double)(1.0f / (float)ns);   /* REAL(4) division, rbsolver.F:246 */
+    const int idiagbeg = c->iarray[ncsp] - ischan + 1;
+    size_t sz = (size_t)ns * nc;
+    double *ydot = malloc(sz * sizeof(double)), *k1 = malloc(sz * sizeof(double));
+    double *k2 = malloc(sz * sizeof(double)), *k3 = malloc(sz * sizeof(double));
+    double *yp = malloc(sz * sizeof(double)), *err = malloc(nc * sizeof(double));
+    double *cc2 = malloc((size_t)(c->iarray[ncsp] + 1) * nc * sizeof(double));
+    double *vdiag = malloc((size_t)ischan * nc * sizeof(double));
+    double dt = DTMIN > DTSTART ? DTMIN : DTSTART;       /* MAX(DTMIN, DTSTART) */
+    double tnow = 0.0;
+    int lpass = 0, rc = 0;
+    long attempts = 0;
+    *nacc = 0; *nrej = 0;
+    while (tnow < chemstep) {
+        if (++attempts > OM_MAX_ATTEMPTS) { rc = -2; break; }
+        om_feval(c, ncsp, nc, rki, y, ydot);
+        double tend = tnow + dt;
+        if (tend > chemstep) { dt = chemstep - tnow; tend = chemstep; }
+        double dtinv = 1.0 / dt;
+        double gdtinv = dtinv * RGAM;
+        om_jacob(c, ncsp, nc, rki, y, cc2);
+        for (int j = idiagbeg; j <= c->iarray[ncsp]; ++j)
+            for (int cc = 0; cc < nc; ++cc) A2(cc2, j, cc) = A2(cc2, j, cc) - gdtinv;
+        om_decomp(c, ncsp, nc, cc2, vdiag);
+        for (size_t q = 0; q < sz; ++q) k1[q] = -ydot[q];
+        om_solve(c, ncsp, nc, cc2, vdiag, k1);
+        for (size_t q = 0; q < sz; ++q) yp[q] = y[q] + k1[q];
+        om_feval(c, ncsp, nc, rki, yp, ydot);
+        double x1 = C21 * dtinv;
+        for (size_t q = 0; q < sz; ++q) k2[q] = -ydot[q] - x1 * k1[q];
+        om_solve(c, ncsp, nc, cc2, vdiag, k2);
+        x1 = C31 * dtinv;
+        double x2 = C32 * dtinv;
+        for (size_t q = 0; q < sz; ++q) k3[q] = -ydot[q] - x1 * k1[q] - x2 * k2[q];
+        om_solve(c, ncsp, nc, cc2, vdiag, k3);
+        for (size_t q = 0; q < sz; ++q) yp[q] = y[q] + B1 * k1[q] + B2 * k2[q] + B3 * k3[q];
+        for (int cc = 0; cc < nc; ++cc) err[cc] = 0.0;
+        for (int n = 0; n < ns; ++n)
+            for (int cc = 0; cc < nc; ++cc) {
+                double ytol = atol[n] + rtol[n] * fabs(A2(yp, n, cc));
+                double e = (D1 * A2(k1, n, cc) + D2 * A2(k2, n, cc) + D3 * A2(k3, n, cc)) / ytol;
+                err[cc] = err[cc] + e * e;
+            }
+        double maxerr = 0.0;
+        for (int cc = 0; cc < nc; ++cc) {
+            double s = sqrt(err[cc] * rnspec);
+            if (UROUND > maxerr) maxerr = UROUND;
+            if (!(s <= maxerr)) maxerr = s;          /* a NaN error rejects the step */
+        }
+        double dtfac = 0.9 / pow(maxerr, GROW);
+        if (maxerr <= 1.0) {
+            (*nacc)++;
+            for (size_t q = 0; q < sz; ++q) y[q] = yp[q] > CONMIN ? yp[q] : CONMIN;
+            tnow = tend;
+            double hi = lpass ? FACMAX : FACONE;
+            if (dtfac > hi) dtfac = hi;
+            if (dtfac < FACMIN) dtfac = FACMIN;
+            double dn = dtfac * dt;
+            if (dn < DTMIN) dn = DTMIN;
+            dt = dn < DTMAX ? dn : DTMAX;
+            lpass = 1;
+        } else {
+            (*nrej)++;
+            dt = FACMIN * dt;
+            lpass = 0;
+            if (dt < DTMIN) {
+                if (fail) { fail->maxerr = maxerr; fail->dt = dt; fail->tnow = tnow; }
+                rc = -1;
+                break;
+            }
+        }
+    }
+    free(ydot); free(k1); free(k2); free(k3); free(yp); free(err); free(cc2); free(vdiag);
+    return rc;
+}
+
+int om_run(const om_ctx *c, int64_t ncells, int blksize, double chemstep,
+           const double *temp, const double *pres, const double *h2o,
+           const double *rj, const double *het, const double *seawater,
+           double *y, const double *rtol, const double *atol,
+           int32_t *nacc, int32_t *nrej, om_fail *fail)
+{
+    const om_tables *t = &c->t;
+    const int ns = c->ns, nr = c->nr;
+    if (fail) { fail->cell = -1; fail->maxerr = 0; fail->dt = 0; fail->tnow = 0; }
+    if (ncells <= 0) return 0;
+    if (blksize < 1) return -3;
+    /* block partition (rbinit.F:286-303) */
+    int64_t nblks = 1 + (ncells - 1) / blksize;
+    int64_t iavg = 1 + (ncells - 1) / nblks;
+    if (iavg > blksize) iavg = blksize;
+    double *rtn = malloc(ns * sizeof(double)), *atn = malloc(ns * sizeof(double));
+    for (int i = 1; i <= ns; ++i) { rtn[i - 1] = rtol[c->inew2old[i] - 1]; atn[i - 1] = atol[c->inew2old[i] - 1]; }
+    int bmax = (int)(ncells - (nblks - 1) * iavg);
+    if (bmax < iavg) bmax = (int)iavg;
+    double *bt = malloc(bmax * sizeof(double)), *bp = malloc(bmax * sizeof(double));
+    double *bh = malloc(bmax * sizeof(double)), *bs = malloc(bmax * sizeof(double));
+    double *brj = malloc((size_t)(t->nphotab ? t->nphotab : 1) * bmax * sizeof(double));
+    double *bhet = malloc((size_t)(t->nhetero ? t->nhetero : 1) * bmax * sizeof(double));
+    double *brk = malloc((size_t)nr * bmax * sizeof(double));
+    double *by = malloc((size_t)ns * bmax * sizeof(double));
+    int rc = 0;
+    for (int64_t b = 0; b < nblks && rc == 0; ++b) {
+        int64_t off = b * iavg;
+        int nc = (int)((b == nblks - 1) ? ncells - off : iavg);
+        for (int cc = 0; cc < nc; ++cc) {
+            bt[cc] = temp[off + cc]; bp[cc] = pres[off + cc]; bh[cc] = h2o[off + cc];
+            bs[cc] = seawater ? seawater[off + cc] : 0.0;
+            for (int i = 0; i < t->nphotab; ++i) brj[(size_t)i * nc + cc] = rj[(size_t)i * ncells + off + cc];
+            for (int i = 0; i < t->nhetero; ++i) bhet[(size_t)i * nc + cc] = het ? het[(size_t)i * ncells + off + cc] : 0.0;
+            for (int iold = 1; iold <= ns; ++iold) {
+                by[(size_t)(c->iold2new[iold] - 1) * nc + cc] = y[(size_t)(iold - 1) * ncells + off + cc];
+            }
+        }
+        int sunlit = om_rates_block(c, nc, bt, bp, bh, brj, bhet, bs, brk);
+        int ncsp = sunlit ? 1 : 2;
+        int32_t na = 0, nj = 0;
+        om_fail f1;
+        rc = om_rbsolver(c, ncsp, nc, chemstep, brk, by, rtn, atn, &na, &nj, &f1);
+        if (rc != 0 && fail) { *fail = f1; fail->cell = off; }
+        for (int cc = 0; cc < nc; ++cc) {
+            if (nacc) nacc[off + cc] = na;
+            if (nrej) nrej[off + cc] = nj;
+            for (int inew = 1; inew <= ns; ++inew)
+                y[(size_t)(c->inew2old[inew] - 1) * ncells + off + cc] = by[(size_t)(inew - 1) * nc + cc];
+        }
+    }
+    free(rtn); free(atn); free(bt); free(bp); free(bh); free(bs); free(brj); free(bhet); free(brk); free(by);
+    return rc;
+}
