@@ -58,6 +58,9 @@ struct cmaq_ros3_ctx {
     // staging for host-pointer runs
     void *d_io = nullptr;
     size_t io_bytes = 0;
+    // kernel timing of the last run: rates start/end, solve end
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    double rates_ms = 0.0, solve_ms = 0.0;
     std::string err;
 };
 
@@ -367,6 +370,7 @@ extern "C" int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech, c
         c->d_fail_info = (double *)((char *)c->d_small + 16);
         c->d_fail_flag = (int32_t *)((char *)c->d_small + 40);
         c->d_fail_code = (int32_t *)((char *)c->d_small + 44);
+        for (hipEvent_t &e : c->ev) INIT_TRY(hipEventCreate(&e));
         INIT_TRY(hipMemcpy(c->d_ops, c->ops.data(), c->ops.size() * sizeof(RateOp), hipMemcpyHostToDevice));
         if (!c->mul.empty())
             INIT_TRY(hipMemcpy(c->d_mul, c->mul.data(), c->mul.size() * sizeof(RateMul), hipMemcpyHostToDevice));
@@ -389,6 +393,8 @@ extern "C" void cmaq_ros3_free(cmaq_ros3_ctx *c)
     hipSetDevice(c->device);
     hipFree(c->d_ops); hipFree(c->d_mul); hipFree(c->d_rtol); hipFree(c->d_atol);
     hipFree(c->d_ws); hipFree(c->d_small); hipFree(c->d_rki); hipFree(c->d_io);
+    for (hipEvent_t e : c->ev)
+        if (e) hipEventDestroy(e);
     delete c;
 }
 
@@ -411,6 +417,14 @@ extern "C" int cmaq_ros3_query(const cmaq_ros3_ctx *c, const char *key, int64_t 
     else if (!std::strcmp(key, "grid_waves")) *value = c->waves;
     else if (!std::strcmp(key, "workspace_doubles_per_lane")) *value = k->nws;
     else return CMAQ_ROS3_E_ARG;
+    return 0;
+}
+
+extern "C" int cmaq_ros3_last_timing(const cmaq_ros3_ctx *c, double *rates_ms, double *solve_ms)
+{
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (rates_ms) *rates_ms = c->rates_ms;
+    if (solve_ms) *solve_ms = c->solve_ms;
     return 0;
 }
 
@@ -474,9 +488,11 @@ extern "C" int cmaq_ros3_run_device(cmaq_ros3_ctx *c, int64_t n, double chemstep
             return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of rate constants failed");
         c->rki_cells = n;
     }
+    HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
+    HIP_TRY(c, hipEventRecord(c->ev[0], s));
     int rc = launch_rates(c, n, temp, pres, h2o, rj, het, sea, c->d_rki, s);
     if (rc) return rc;
-    HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
+    HIP_TRY(c, hipEventRecord(c->ev[1], s));
     SolveParams P;
     P.ncells = n; P.chemstep = chemstep; P.rki = c->d_rki; P.y = y; P.nacc = nacc; P.nrej = nrej;
     P.rtol = c->d_rtol; P.atol = c->d_atol; P.ws = c->d_ws; P.queue = c->d_queue;
@@ -484,9 +500,15 @@ extern "C" int cmaq_ros3_run_device(cmaq_ros3_ctx *c, int64_t n, double chemstep
     P.fail_code = c->d_fail_code; P.max_attempts = c->max_attempts; P.refill_min = c->refill_min;
     void *args[] = {&P};
     HIP_TRY(c, hipLaunchKernel(c->mk->solve_kernel, dim3(c->grid_blocks), dim3(kSolveBlock), args, 0, s));
+    HIP_TRY(c, hipEventRecord(c->ev[2], s));
     unsigned char small[64];
     HIP_TRY(c, hipMemcpyAsync(small, c->d_small, 64, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
+    float ms_r = 0.f, ms_s = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms_r, c->ev[0], c->ev[1]));
+    HIP_TRY(c, hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
+    c->rates_ms = ms_r;
+    c->solve_ms = ms_s;
     int32_t flag, code;
     int64_t cell;
     double info[3];

@@ -147,3 +147,81 @@ def make_cells(mech, cells, seed=0, workload="clean", nlay_cells=None, svoc_ppb=
     rj = clear_sky_j(mech)[:, None] * scale[None, :]
     return {"temp": temp, "pres": pres, "h2o": h2o, "rj": np.ascontiguousarray(rj),
             "y": np.ascontiguousarray(y), "night": night}
+
+
+# ---- the same generator on the GPU (bench-sized workloads) ----------------
+def _i64(c):
+    """uint64 constant as the int64 with the same bits."""
+    return c - (1 << 64) if c >= (1 << 63) else c
+
+
+def _t_splitmix(x):
+    import torch  # noqa: F401
+    x = x + _i64(0x9E3779B97F4A7C15)
+    z = x
+    z = (z ^ ((z >> 30) & ((1 << 34) - 1))) * _i64(0xBF58476D1CE4E5B9)
+    z = (z ^ ((z >> 27) & ((1 << 37) - 1))) * _i64(0x94D049BB133111EB)
+    return z ^ ((z >> 31) & ((1 << 33) - 1))
+
+
+def uniform_torch(seed, field, cells):
+    """Bit-identical to uniform() for int64 torch tensors of cell indices."""
+    import torch
+    with np.errstate(over="ignore"):
+        key = int(_splitmix(np.uint64(seed) * np.uint64(0x100000001B3) + np.uint64(field)))
+    h = _t_splitmix(torch.bitwise_xor(cells * _i64(0xD6E8FEB86659FD93), _i64(key)))
+    return ((h >> 11) & ((1 << 53) - 1)).to(torch.float64) * (1.0 / 9007199254740992.0)
+
+
+def make_cells_torch(mech, ncells, seed=0, workload="clean", nlay_cells=None, svoc_ppb=None,
+                     device="cuda"):
+    """make_cells for cells 0..ncells-1, generated directly in device memory."""
+    import torch
+    cells = torch.arange(ncells, dtype=torch.int64, device=device)
+    names = mech["chemistry_spc"]
+    ns = len(names)
+    logu = lambda u, lo, hi: torch.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))
+    out = {
+        "temp": 250.0 + 60.0 * uniform_torch(seed, F_T, cells),
+        "pres": (500.0 + 513.0 * uniform_torch(seed, F_P, cells)) * 100.0 / 101325.0,
+        "h2o": logu(uniform_torch(seed, F_H2O, cells), 1.0e3, 3.0e4),
+    }
+    y = torch.empty((ns, ncells), dtype=torch.float64, device=device)
+    fast = set(SUB_DTMIN_SPECIES.get(mech["mechname"], []))
+    for i, nm in enumerate(names):
+        u = uniform_torch(seed, F_SPC + i, cells)
+        if nm in fast:
+            y[i] = 1.0e-6 * 1.0e-3
+            continue
+        if nm == "O3":
+            v = 20.0 + 60.0 * u
+        elif nm == "CO":
+            v = 80.0 + 220.0 * u
+        elif svoc_ppb is not None and nm.startswith("VROC"):
+            v = logu(u, *svoc_ppb)
+        else:
+            v = logu(u, 1.0e-6, 1.0e2)
+        y[i] = v * 1.0e-3
+        del u, v
+    cosz = uniform_torch(seed, F_COSZ, cells)
+    night = uniform_torch(seed, F_NIGHT, cells) < 0.30
+    if workload == "mixed":
+        reg = uniform_torch(seed, F_REGIME, cells)
+        night = (reg >= 0.40) & (reg < 0.70)
+        source = reg >= 0.70
+        idx = {nm: i for i, nm in enumerate(names)}
+        for nm, fld in (("NO", F_NO), ("NO2", F_NO2)):
+            v = logu(uniform_torch(seed, fld, cells), 10.0, 500.0) * 1.0e-3
+            y[idx[nm]] = torch.where(source, v, y[idx[nm]])
+        for nm in VOC_SPECIES.get(mech["mechname"], []):
+            if nm in idx:
+                y[idx[nm]] = torch.where(source, y[idx[nm]] * 10.0, y[idx[nm]])
+    scale = torch.where(night, torch.zeros_like(cosz), cosz)
+    if workload == "regional":
+        k = torch.div(cells, int(nlay_cells), rounding_mode="floor").to(torch.float64)
+        scale = scale * torch.exp(-k / 20.0)
+    jprof = torch.as_tensor(clear_sky_j(mech), device=device)
+    out["rj"] = (jprof[:, None] * scale[None, :]).contiguous()
+    out["y"] = y
+    out["night"] = night
+    return out

@@ -79,6 +79,7 @@ def lib():
         L.cmaq_ros3_last_error.argtypes = [vp]
         L.cmaq_ros3_last_error.restype = C.c_char_p
         L.cmaq_ros3_free.argtypes = [vp]
+        L.cmaq_ros3_last_timing.argtypes = [vp, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -178,6 +179,12 @@ class Ros3Solver:
         if rc != 0:
             raise self._err(rc)
         return v.value
+
+    def last_timing(self):
+        """(rates_ms, solve_ms) of the last run, from HIP events on its stream."""
+        a, b = C.c_double(), C.c_double()
+        lib().cmaq_ros3_last_timing(self._ctx, C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def species_order(self):
         out = np.zeros(self.ns, np.int32)

@@ -140,6 +140,13 @@ int cmaq_ros3_rates_device(cmaq_ros3_ctx *ctx, int64_t ncells,
                            const double *het_per_min, const double *seawater,
                            double *rki, void *stream);
 
+/*
+ * Device time of the kernels of the last cmaq_ros3_run / _run_device call on
+ * this context, measured with HIP events on the call's stream:
+ * rates_ms = CALC_RCONST kernel, solve_ms = persistent ROS3 kernel.
+ */
+int cmaq_ros3_last_timing(const cmaq_ros3_ctx *ctx, double *rates_ms, double *solve_ms);
+
 /* Sizes of the selected pattern: key in {"ischan","nnz_day","nnz_night",
  * "lu_ops_day","lu_ops_night","nrxns","numb_mech_spc","grid_waves"}. */
 int cmaq_ros3_query(const cmaq_ros3_ctx *ctx, const char *key, int64_t *value);

@@ -1,0 +1,196 @@
+"""Golden rate-constant vectors from the reference's generated CALC_RCONST.
+
+Reads MECHS/<mech>/RXNS_FUNC_MODULE.F90 from the reference tree, translates
+the chemmech-generated Fortran statements of CALC_RCONST and of its rate
+functions (POWER_T02, ARRHENUIS_T03/T04, FALLOFF_T08..T11, HALOGEN_FALLOFF)
+into Python expression by expression, keeping the Fortran evaluation order,
+and evaluates them for seeded cells.  Python's float arithmetic, math.exp,
+math.log10 and ** are the same glibc double routines a gfortran build of the
+reference calls, so the expected values are the reference's own rate
+constants.  The table-driven evaluators (the oracle and the GPU kernel) are
+checked against these vectors; they never see this code.
+
+    python tests/golden/make_rate_golden.py [reference_root]
+
+writes tests/golden/rates_<mech>.json (inputs and expected RKI).
+"""
+
+import json
+import math
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MECHS = {"cb6r5_ae7_aq": "CCTM/src/MECHS/cb6r5_ae7_aq/RXNS_FUNC_MODULE.F90",
+         "cracmm2": "CCTM/src/MECHS/cracmm2/RXNS_FUNC_MODULE.F90"}
+
+
+def _statements(lines):
+    out, buf = [], ""
+    for raw in lines:
+        s = raw.split("!", 1)[0].rstrip()
+        st = s.strip()
+        if buf and st.startswith("&"):
+            st = st[1:]
+        if st.endswith("&"):
+            buf += st[:-1] + " "
+            continue
+        buf += st
+        if buf.strip():
+            out.append(buf.strip())
+        buf = ""
+    return out
+
+
+def _expr(e):
+    e = re.sub(r"RKI\(\s*NCELL\s*,\s*(\d+)\s*\)", r"RKI[\1]", e)
+    e = re.sub(r"RJBLK\(\s*NCELL\s*,\s*IJ_(\w+)\s*\)", r'J["\1"]', e)
+    e = re.sub(r"BLKHET\(\s*NCELL\s*,\s*IK_(\w+)\s*\)", r'H["\1"]', e)
+    e = re.sub(r"SEAWATER\s*\(\s*NCELL\s*\)", "SEA", e)
+    e = re.sub(r"BLKPRES\(\s*NCELL\s*\)", "PRES_", e)
+    e = re.sub(r"BLKTEMP\(\s*NCELL\s*\)", "TEMP_", e)
+    e = re.sub(r"BLKH2O\(\s*NCELL\s*\)", "H2O_", e)
+    e = re.sub(r"(\d+\.?\d*)[dD]([+-]?\d+)", r"\1e\2", e)
+    e = re.sub(r"\bDEXP\(", "math.exp(", e)
+    e = re.sub(r"\bDLOG10\(", "math.log10(", e)
+    e = re.sub(r"\bDMIN1\s*\(", "min(", e)
+    return e
+
+
+def _functions(stmts):
+    """Python source for every REAL(8) FUNCTION of the module."""
+    src, i = [], 0
+    while i < len(stmts):
+        m = re.match(r"REAL\(\s*8\s*\)\s+FUNCTION\s+(\w+)\s*\((.*)\)", stmts[i])
+        if not m:
+            i += 1
+            continue
+        name, args = m.group(1), [a.strip() for a in m.group(2).split(",")]
+        body = []
+        i += 1
+        while not stmts[i].startswith("END FUNCTION"):
+            st = stmts[i]
+            a = re.match(r"(\w+)\s*=\s*(.+)$", st)
+            if a and not re.match(r"(REAL|INTEGER|INTRINSIC|IMPLICIT)", st):
+                body.append(f"    {a.group(1)} = {_expr(a.group(2))}")
+            i += 1
+        src.append(f"def {name}({', '.join(args)}):")
+        src += body
+        src.append(f"    return {name}")
+        src.append("")
+    return "\n".join(src)
+
+
+def translate(path):
+    lines = open(path).read().splitlines()
+    stmts = _statements(lines)
+    funcs = _functions(stmts)
+    b = next(i for i, s in enumerate(stmts) if s.startswith("SUBROUTINE CALC_RCONST"))
+    e = next(i for i, s in enumerate(stmts) if s.startswith("END SUBROUTINE CALC_RCONST"))
+    body = stmts[b + 1:e]
+    params = {}
+    py = ["def calc(TEMP_, PRES_, H2O_, J, H, SEA, sunlit, NR, MULT):",
+          "    RKI = [0.0] * (NR + 1)"]
+    indent = "    "
+    depth_sun = None
+    i = 0
+    while i < len(body):
+        st = body[i]
+        m = re.match(r"REAL\(\s*8\s*\),\s*PARAMETER\s*::\s*(\w+)\s*=\s*(.+)$", st)
+        if m:
+            params[m.group(1)] = _expr(m.group(2))
+            py.append(f"    {m.group(1)} = {_expr(m.group(2))}")
+            i += 1
+            continue
+        if re.match(r"IF\(\s*LSUNLIGHT\s*\)\s*THEN", st):
+            py.append("    if sunlit:")
+            indent = "        "
+            depth_sun = 0
+            i += 1
+            continue
+        m = re.match(r"IF\s*\(\s*SEAWATER\s*\(NCELL\)\s*\.GT\.\s*(\S+)\s*\)\s*THEN", st)
+        if m:
+            py.append(f"{indent}if SEA > {_expr(m.group(1))}:")
+            indent += "    "
+            depth_sun += 1
+            i += 1
+            continue
+        if st == "ELSE" and depth_sun:
+            py.append(f"{indent[:-4]}else:")
+            i += 1
+            continue
+        if st.startswith("END IF") and depth_sun is not None:
+            if depth_sun > 0:
+                depth_sun -= 1
+                indent = indent[:-4]
+            else:
+                depth_sun = None
+                indent = "    "
+            i += 1
+            continue
+        m = re.match(r"IF\s*\(\s*(NW\w+)\s*\.GT\.\s*0\s*\)\s*THEN", st)
+        if m:
+            # multiplier block: DO NRT / IRXN = NRXW..(NRT) / RKI(NCELL,IRXN) = RKI * factor
+            blk = []
+            i += 1
+            while not body[i].startswith("END IF"):
+                blk.append(body[i])
+                i += 1
+            lst = next(re.match(r"IRXN\s*=\s*(\w+)\(", s).group(1) for s in blk if s.startswith("IRXN"))
+            fac = next(re.match(r"RKI\(\s*NCELL\s*,\s*IRXN\s*\)\s*=\s*RKI\(\s*NCELL\s*,\s*IRXN\s*\)\s*\*\s*(.+)$", s).group(1)
+                       for s in blk if s.startswith("RKI"))
+            py.append(f"    for r in MULT['{lst.lower()}']:")
+            py.append(f"        RKI[r] = RKI[r] * {_expr(fac)}")
+            i += 1
+            continue
+        m = re.match(r"(\w+(?:\(\s*NCELL\s*,\s*\d+\s*\))?)\s*=\s*(.+)$", st)
+        if m and m.group(1) != "RKI" and not re.match(r"(REAL|INTEGER|LOGICAL|IMPLICIT|USE)\b", st):
+            lhs = m.group(1)
+            if lhs.startswith("RKI"):
+                lhs = _expr(lhs)
+            py.append(f"{indent}{lhs} = {_expr(m.group(2))}")
+        i += 1
+    py.append("    return RKI[1:]")
+    return funcs + "\n" + "\n".join(py) + "\n"
+
+
+def main():
+    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from cmaq_amd.inputs import make_cells
+    from cmaq_amd.ros3 import load_mechanism
+    for name, rel in MECHS.items():
+        mech = load_mechanism(name)
+        src = translate(os.path.join(ref, rel))
+        # RXNS_DATA parameters used by the multiplier loops
+        ns = {"math": math, "ATM_AIR": mech["atm_air"], "ATM_O2": mech["atm_o2"], "ATM_N2": mech["atm_n2"],
+              "ATM_CH4": mech["atm_ch4"], "ATM_H2": mech["atm_h2"]}
+        exec(compile(src, f"<{name} CALC_RCONST>", "exec"), ns)
+        ncell = 48
+        d = make_cells(mech, np.arange(ncell), seed=11)
+        rng = np.random.default_rng(5)
+        het = rng.uniform(0.0, 1.0e-2, size=(mech["nhetero"], ncell))
+        sea = np.where(rng.uniform(size=ncell) < 0.5, 0.0, rng.uniform(0.0, 1.0, size=ncell))
+        mult = {k: v for k, v in mech.items() if k.startswith("nrxw")}
+        rki = np.zeros((mech["nrxns"], ncell))
+        for c in range(ncell):
+            J = {n: float(d["rj"][k, c]) for k, n in enumerate(mech["photab"])}
+            H = {n: float(het[k, c]) for k, n in enumerate(mech["hetero"])}
+            sunlit = bool((d["rj"][:, c] > 0).any())
+            rki[:, c] = ns["calc"](float(d["temp"][c]), float(d["pres"][c]), float(d["h2o"][c]), J, H,
+                                   float(sea[c]), sunlit, mech["nrxns"], mult)
+        out = {"mechanism": mech["mechname"], "source": rel, "ncells": ncell,
+               "temp": d["temp"].tolist(), "pres_atm": d["pres"].tolist(), "h2o_ppm": d["h2o"].tolist(),
+               "rj": d["rj"].tolist(), "het": het.tolist(), "seawater": sea.tolist(),
+               "rki_hex": [[float(v).hex() for v in row] for row in rki]}
+        path = os.path.join(HERE, f"rates_{name}.json")
+        with open(path, "w") as f:
+            json.dump(out, f)
+        print(f"{name}: {ncell} cells -> {path}")
+
+
+if __name__ == "__main__":
+    main()
