@@ -68,6 +68,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run `python -m cmaq_amd.build` first")
+        # One HIP runtime per process: when PyTorch is present its bundled
+        # libamdhip64 (same SONAME) must be loaded first so that the solver
+        # binds to it and shares device pointers and streams with torch.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         L.cmaq_ros3_init.argtypes = [C.POINTER(vp), C.POINTER(_Mech), _f64p, _f64p, C.POINTER(_Opts)]
@@ -212,8 +219,11 @@ class Ros3Solver:
         return yy, na, nj
 
     def run_device(self, chemstep, temp, pres_atm, h2o_ppm, rj, y, nacc=None, nrej=None,
-                   het=None, seawater=None, stream=None):
-        """torch CUDA tensors (float64; nacc/nrej int32); y updated in place."""
+                   het=None, seawater=None, stream=None, raise_on_fail=True):
+        """torch CUDA tensors (float64; nacc/nrej int32); y updated in place.
+
+        With raise_on_fail=False a convergence failure (cells marked by
+        nacc = -1) returns the failure record instead of raising."""
         import torch
         n = temp.numel()
         for t in (temp, pres_atm, h2o_ppm, rj, y):
@@ -227,8 +237,10 @@ class Ros3Solver:
         rc = lib().cmaq_ros3_run_device(self._ctx, n, chemstep, _ptr(temp), _ptr(pres_atm), _ptr(h2o_ppm),
                                         _ptr(rj), _ptr(het), _ptr(seawater), _ptr(y), _ptr(nacc), _ptr(nrej),
                                         C.byref(f), C.c_void_p(stream))
-        if rc != 0:
-            raise self._err(rc, {"cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow})
+        fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, fail)
+        return fail if rc != 0 else None
 
     def rates_device(self, temp, pres_atm, h2o_ppm, rj, rki, het=None, seawater=None, stream=None):
         import torch

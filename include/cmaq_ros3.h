@@ -110,6 +110,10 @@ int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech,
  * 0), seawater [ncells] (may be NULL = 0), y [numb_mech_spc][ncells] in/out.
  * n_accepted / n_rejected (may be NULL) receive the per-cell step counts that
  * the reference's rbstats build reports (rbsolver.F:462-464, 521-524).
+ * Where the reference would stop the run (DT below DTMIN, rbsolver.F:528-583)
+ * this call integrates every other cell, sets n_accepted = -1 for each
+ * failing cell, leaves that cell's y at its last accepted state and returns
+ * CMAQ_ROS3_E_CONVERGENCE with the first failure in *fail.
  */
 int cmaq_ros3_run(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min,
                   const double *temp_k, const double *pres_atm,

@@ -722,7 +722,7 @@ int om_run(const om_ctx *c, int64_t ncells, int blksize, double chemstep,
     double *brk = malloc((size_t)nr * bmax * sizeof(double));
     double *by = malloc((size_t)ns * bmax * sizeof(double));
     int rc = 0;
-    for (int64_t b = 0; b < nblks && rc == 0; ++b) {
+    for (int64_t b = 0; b < nblks; ++b) {
         int64_t off = b * iavg;
         int nc = (int)((b == nblks - 1) ? ncells - off : iavg);
         for (int cc = 0; cc < nc; ++cc) {
@@ -738,8 +738,12 @@ int om_run(const om_ctx *c, int64_t ncells, int blksize, double chemstep,
         int ncsp = sunlit ? 1 : 2;
         int32_t na = 0, nj = 0;
         om_fail f1;
-        rc = om_rbsolver(c, ncsp, nc, chemstep, brk, by, rtn, atn, &na, &nj, &f1);
-        if (rc != 0 && fail) { *fail = f1; fail->cell = off; }
+        int brc = om_rbsolver(c, ncsp, nc, chemstep, brk, by, rtn, atn, &na, &nj, &f1);
+        if (brc != 0) {
+            if (rc == 0 && fail) { *fail = f1; fail->cell = off; }
+            if (rc == 0) rc = brc;
+            na = -1;                          /* marks the cells of a failed block */
+        }
         for (int cc = 0; cc < nc; ++cc) {
             if (nacc) nacc[off + cc] = na;
             if (nrej) nrej[off + cc] = nj;

@@ -88,8 +88,9 @@ int om_rbsolver(const om_ctx *c, int ncsp, int ncells, double chemstep,
 
 /* Whole-domain driver: blocks of `blksize` cells (1 = per-cell step
  * control), y in MECHANISM order [ns][ncells] in/out, per-cell counts.
- * rtol/atol are per species in mechanism order.  Returns 0 or the
- * (negative) failure code. */
+ * rtol/atol are per species in mechanism order.  Every block is
+ * integrated; a block whose step fell below DTMIN gets nacc = -1 and the
+ * first failure is returned (negative code) and described in *fail. */
 int om_run(const om_ctx *c, int64_t ncells, int blksize, double chemstep,
            const double *temp, const double *pres_atm, const double *h2o_ppm,
            const double *rj, const double *het, const double *seawater,
