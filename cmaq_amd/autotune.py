@@ -1,0 +1,69 @@
+"""Autotuner for the generated ROS3 kernels.
+
+For one mechanism, times every built code-generation variant (register budget
+and load lookahead, which fix the occupancy) and a few refill thresholds of
+the persistent kernel on a synthetic workload resident in HBM, and records
+the fastest in a JSON table (cmaq_amd/tuning.json); the build marks that
+variant as the default.
+
+    python -m cmaq_amd.autotune --mech cb6r5_ae7_aq --ncells 1048576 --out tuning.json
+"""
+
+import argparse
+import json
+import os
+import sys
+
+
+def tune(mech_name, ncells, workload="regional", seed=1, refills=(1, 2, 4), log=print):
+    import torch
+    from .inputs import make_cells_torch
+    from .ros3 import Ros3Solver, load_mechanism
+    mech = load_mechanism(mech_name)
+    dev = torch.device("cuda:0")
+    d = make_cells_torch(mech, ncells, seed=seed, workload=workload,
+                         nlay_cells=max(1, ncells // 35), device=dev)
+    y = torch.empty_like(d["y"])
+    na = torch.zeros(ncells, dtype=torch.int32, device=dev)
+    nj = torch.zeros(ncells, dtype=torch.int32, device=dev)
+    results = []
+    base = Ros3Solver(mech)
+    for var in base.variants():
+        for rf in refills:
+            s = Ros3Solver(mech, refill_min=rf)
+            s.set_variant(var)
+            best = None
+            for _ in range(2):
+                y.copy_(d["y"])
+                s.run_device(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], y, na, nj, raise_on_fail=False)
+                ms = s.last_timing()[1]
+                best = ms if best is None else min(best, ms)
+            att = int((na.clamp(min=0) + nj).sum())
+            rate = att / (best / 1000.0)
+            results.append({"variant": var, "refill_min": rf, "solve_ms": best, "cell_steps_per_s": rate})
+            log(f"[autotune] {mech_name} {var} refill {rf}: {best:.1f} ms, {rate / 1e6:.2f} M cell-steps/s")
+            s.close()
+    best = max(results, key=lambda r: r["cell_steps_per_s"])
+    return {"variant": best["variant"], "refill_min": best["refill_min"], "ncells": ncells,
+            "workload": workload, "results": results}
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--mech", action="append", default=None)
+    p.add_argument("--ncells", type=int, default=1048576)
+    p.add_argument("--out", default="tuning.json")
+    a = p.parse_args()
+    table = {}
+    if os.path.exists(a.out):
+        with open(a.out) as f:
+            table = json.load(f)
+    for m in a.mech or ["cb6r5_ae7_aq"]:
+        table[m] = tune(m, a.ncells, log=lambda s: print(s, file=sys.stderr, flush=True))
+        with open(a.out, "w") as f:
+            json.dump(table, f, indent=1)
+    print(json.dumps({m: {k: v for k, v in t.items() if k != "results"} for m, t in table.items()}))
+
+
+if __name__ == "__main__":
+    main()

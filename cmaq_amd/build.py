@@ -18,6 +18,14 @@ GEN = os.path.join(CSRC, "gen")
 OBJ = os.path.join(GEN, "obj")
 LIB = os.path.join(HERE, "libcmaq_ros3.so")
 MECHS = {"cb6r5_ae7_aq": "cb6r5_ae7_aq.json", "cracmm2": "cracmm2.json"}
+# code-generation variants per mechanism: (register budget, load lookahead);
+# the first is the default unless tuning.json names another (written by
+# cmaq_amd/autotune.py from timings on the GPU)
+VARIANTS = {
+    "cb6r5_ae7_aq": [(110, 32), (100, 32), (56, 32)],
+    "cracmm2": [(110, 32), (100, 32)],
+}
+TUNING = os.path.join(HERE, "tuning.json")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -41,22 +49,40 @@ def _write_if_changed(path, text):
     return True
 
 
+def tuned(name):
+    """(default variant, its refill threshold) for a mechanism."""
+    table = {}
+    if os.path.exists(TUNING):
+        with open(TUNING) as f:
+            table = json.load(f)
+    b, l = VARIANTS[name][0]
+    entry = table.get(name, {})
+    return entry.get("variant", f"b{b}_l{l}"), int(entry.get("refill_min", 2))
+
+
 def generate(names=None):
     os.makedirs(GEN, exist_ok=True)
     names = names or list(MECHS)
-    srcs = []
+    srcs, cnames = [], []
     for name in names:
-        text, _ = emit(load_mech(name), name)
-        path = os.path.join(GEN, f"ros3_{name}.hip")
-        _write_if_changed(path, text)
-        srcs.append(path)
+        mech = load_mech(name)
+        dflt, refill = tuned(name)
+        for (b, l) in VARIANTS[name]:
+            var = f"b{b}_l{l}"
+            cname = f"{name}_{var}"
+            text, _ = emit(mech, cname, budget=b, lookahead=l, variant=var, is_default=(var == dflt),
+                           refill_min=refill)
+            path = os.path.join(GEN, f"ros3_{cname}.hip")
+            _write_if_changed(path, text)
+            srcs.append(path)
+            cnames.append(cname)
     reg = ["// Generated by cmaq_amd/build.py; do not edit.", "#include \"../ros3_registry.h\""]
-    for name in names:
+    for name in cnames:
         reg.append(f"extern \"C\" const ros3::MechKernels ros3_kernels_{name};")
     reg.append("extern \"C\" const ros3::MechKernels *const ros3_mech_registry[] = {")
-    reg += [f"    &ros3_kernels_{name}," for name in names]
+    reg += [f"    &ros3_kernels_{name}," for name in cnames]
     reg.append("};")
-    reg.append(f"extern \"C\" const int ros3_mech_registry_size = {len(names)};")
+    reg.append(f"extern \"C\" const int ros3_mech_registry_size = {len(cnames)};")
     path = os.path.join(GEN, "registry.cpp")
     _write_if_changed(path, "\n".join(reg) + "\n")
     srcs.append(path)
@@ -64,7 +90,7 @@ def generate(names=None):
 
 
 def _compile(src, obj, host_only=False):
-    deps = [src, os.path.join(CSRC, "ros3_device.h"), os.path.join(CSRC, "ros3_solver_kernel.inc"),
+    deps = [src, os.path.join(CSRC, "ros3_device.h"), os.path.join(CSRC, "ros3_registry.h"), os.path.join(CSRC, "ros3_solver_kernel.inc"),
             os.path.join(CSRC, "ros3_registry.h"), os.path.join(HERE, "..", "include", "cmaq_ros3.h")]
     if os.path.exists(obj) and all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps if os.path.exists(d)):
         return obj

@@ -1,33 +1,38 @@
 """Emit the fully unrolled gfx950 ROS3 kernels for one mechanism.
 
-The generated translation unit holds, for the mechanism's day pattern:
+The generated translation unit holds, for the mechanism's day pattern,
+straight-line device functions built from programs.py's operation lists:
 
-* ``feval``   -- reaction rates then f(y) in gather form: each species
-  accumulates its loss / production terms in exactly the order in which
-  RBFEVAL scatters them (rbfeval.F:93-149), so the rounding is the
-  reference's;
-* ``jac_lu``  -- Jacobian, diagonal shift and sparse LU fused column by
-  column: the Jacobian entries of column j (built in RBJACOB's scatter order,
-  rbjacob.F:112-183), the shift CC2(jj) -= 1/(gamma*h) (rbsolver.F:377-381),
-  then RBDECOMP's loop-1 updates, the pivot reciprocal and the L scaling
-  (rbdecomp.F:86-136).  Each factored entry is written once;
-* ``solve``   -- forward then backward substitution in RBSOLVE's term order
-  (rbsolve.F:78-187) with the stage right-hand side formed on the fly;
-* the persistent solver kernel instantiated for these functions.
+* ``feval``    -- f(y) in RBFEVAL's order (rbfeval.F:93-149);
+* ``jac_lu``   -- Jacobian, diagonal shift and sparse LU (rbjacob.F,
+  rbsolver.F:377-381, rbdecomp.F);
+* ``solve1..3`` -- the three stage solves (rbsolve.F) with the stage
+  right-hand sides of rbsolver.F:386-422 formed in place;
+* the persistent solver kernel (ros3_solver_kernel.inc) that calls them.
 
-Night cells are integrated with the day pattern: the extra entries are
-exact zeros (their only contributions are photolysis terms with J = 0), so
-every night result is bit-identical to the reference's night pattern.
+Registers are assigned by regalloc.allocate: at most REG_BUDGET doubles of a
+lane's workspace are register-resident, loads are issued LOOKAHEAD operations
+ahead of their use, and every workspace access is a buffer load/store with a
+compile-time scalar offset from the wave's workspace base, so no per-access
+address arithmetic is spent in the vector ALU.
+
+Night cells are integrated with the day pattern: the extra entries are exact
+zeros (their only contributions are photolysis terms with J = 0), so every
+night result is bit-identical to the reference's night pattern.
 """
 
 import struct
 
+from .programs import feval_ops, jac_lu_ops, solve_ops
+from .regalloc import Op, allocate
 from .symbolic import analyse
 
-# fence spacing in the generated code (rates, species, solve rows)
-FEVAL_RATE_GROUP = 16
-FEVAL_SPC_GROUP = 8
-SOLVE_ROW_GROUP = 8
+# register budget (doubles per lane held by a generated function) and load
+# lookahead (operations); the kernel's occupancy target follows the budget
+REG_BUDGET = 48
+LOOKAHEAD = 8
+JAC_ORDER = "column"
+FEVAL_ORDER = "scatter"
 
 HASH_OFFSET = 0xcbf29ce484222325
 HASH_PRIME = 0x100000001b3
@@ -61,25 +66,54 @@ def structure_hash(m):
     return h
 
 
-def _lit(x):
-    s = repr(float(x))
-    if "e" not in s and "." not in s and "inf" not in s and "nan" not in s:
-        s += ".0"
-    return s
+def _function(name, params, items, off, budget, dynamic=None):
+    """HIP source of one generated function from allocator items."""
+    dynamic = dynamic or {}
+    w = [f"__device__ __noinline__ void {name}(double *base_{params})", "{",
+         "    ROS3_PROLOGUE"]
+    for d, src in dynamic.items():
+        w.append(f"    const int {d}_off = __builtin_amdgcn_readfirstlane({src});")
+    w.append("    double " + ", ".join(f"r{i}" for i in range(budget)) + ";")
+
+    def addr(slot):
+        arr, i = slot
+        if arr in dynamic:
+            return f"({arr}_off + {i})"
+        return str(off[arr] + i)
+
+    for it in items:
+        if it[0] == "load":
+            w.append(f"    r{it[1]} = LD({addr(it[2])});")
+        elif it[0] == "store":
+            w.append(f"    ST({addr(it[2])}, r{it[1]});")
+        else:
+            w.append("    " + it[1])
+    w.append("}")
+    return w
 
 
-def emit(m, cname):
+def _stat(ops, counts):
+    return (len(ops), counts.get("load", 0), counts.get("store", 0))
+
+
+def waves_per_eu(budget):
+    """Occupancy the kernel is compiled for: VGPRs ~ 2 per resident double plus
+    ~28 for addresses, loop state and temporaries, out of 512 per lane."""
+    vgprs = 2 * budget + 28
+    vgprs = (vgprs + 7) // 8 * 8
+    return max(1, min(8, 512 // vgprs))
+
+
+def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, refill_min=0):
+    budget = budget or REG_BUDGET
+    lookahead = lookahead or LOOKAHEAD
     sym = analyse(m)
     if sym.ischan != sym.ns:
         raise ValueError("species without Jacobian entries are not supported")
     ns, nr = sym.ns, sym.nr
     day = sym.day
     nnz = day.nnz
-    out = []
-    w = out.append
-    h = structure_hash(m)
 
-    # workspace layout (entries per lane)
     off = {}
     pos = 0
     for name, size in (("RK", nr), ("RATE", nr), ("Y", ns), ("YP", ns), ("F", ns),
@@ -87,152 +121,97 @@ def emit(m, cname):
         off[name] = pos
         pos += size
     nws = pos
+    h = structure_hash(m)
+    stats = {}
+    entry_all = lambda slot: True
+    body = []
+    b = body.append
 
+    # f(y): y at a runtime workspace offset (OFF_Y or OFF_YP)
+    ops = feval_ops(sym, FEVAL_ORDER)
+    items, cnt = allocate(ops, {"F"}, budget, lookahead, entry_all)
+    stats["feval"] = _stat(ops, cnt)
+    body += _function("feval", ", int ys_off_", items, off, budget, {"YS": "ys_off_"})
+    b("")
+
+    ops = jac_lu_ops(sym, JAC_ORDER)
+    items, cnt = allocate(ops, {"LU", "VD"}, budget, lookahead, entry_all)
+    stats["jac_lu"] = _stat(ops, cnt)
+    body += _function("jac_lu", ", double gdtinv", items, off, budget)
+    b("")
+
+    for stage in (1, 2, 3):
+        ops = solve_ops(sym, stage)
+        outs = {f"K{stage}"}
+        if stage == 1:
+            # YP = Y + K1 (rbsolver.F:395-398) as soon as K1(i) is final
+            fused = []
+            for op in ops:
+                fused.append(op)
+                if op.fmt == "{0} = {0} * {1};" and op.args[0][0] == "K1":
+                    i = op.args[0][1]
+                    fused.append(Op("{0} = {1} + {2};", [("YP", i), ("Y", i), ("K1", i)], False))
+            ops = fused
+            outs.add("YP")
+        items, cnt = allocate(ops, outs, budget, lookahead, entry_all)
+        stats[f"solve{stage}"] = _stat(ops, cnt)
+        params = "" if stage == 1 else (", double x1" if stage == 2 else ", double x1, double x2")
+        body += _function(f"solve{stage}", params, items, off, budget)
+        b("")
+
+    out = []
+    w = out.append
     w(f"// Generated by cmaq_amd/codegen/emit_hip.py for {m['mechname']}; do not edit.")
+    w(f"// register budget {budget} doubles, load lookahead {lookahead} operations;")
+    w("// per call (operations, loads, stores): " + ", ".join(f"{k} {v}" for k, v in stats.items()))
     w("#include \"../ros3_device.h\"")
     w(f"namespace ros3 {{ namespace {cname} {{")
     w(f"constexpr int NS = {ns}, NR = {nr}, NNZ = {nnz}, NWS = {nws};")
+    w(f"constexpr int REG_BUDGET = {budget}, WAVES_PER_EU = {waves_per_eu(budget)};")
     w(f"constexpr unsigned long long STRUCT_HASH = 0x{h:016x}ull;")
     for k, v in off.items():
         w(f"constexpr int OFF_{k} = {v};")
-    w("// every per-lane vector is addressed from the lane base pointer wb with a")
-    w("// compile-time offset, so the compiler can tell all of them apart")
-    w("#define AT(off, k) wb[((off) + (k)) * kWave]")
-    w("#define W(p, k) AT(OFF_##p, k)")
-    w("// compiler-only memory fence: bounds how far loads are merged or hoisted,")
-    w("// which keeps the per-lane register footprint (and so occupancy) in check")
-    w("#define FENCE() asm volatile(\"\" ::: \"memory\")")
+    w("// Workspace access: the wave's workspace is one buffer resource; entry k of")
+    w("// the calling lane sits at byte (k * 64 + lane) * 8 from the wave's base.")
+    w("#define ROS3_PROLOGUE \\")
+    w("    double *base = (double *)((((unsigned long long)(unsigned int)__builtin_amdgcn_readfirstlane( \\")
+    w("                                  (int)((unsigned long long)base_ >> 32))) << 32) | \\")
+    w("                              (unsigned long long)(unsigned int)__builtin_amdgcn_readfirstlane( \\")
+    w("                                  (int)(unsigned long long)base_)); \\")
+    w("    const __amdgpu_buffer_rsrc_t rs = \\")
+    w("        __builtin_amdgcn_make_buffer_rsrc(base, (short)0, NWS * kWave * 8, 0x00020000); \\")
+    w("    const int vo = __lane_id() * 8;")
+    w("#define LD(k) __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, (k) * (kWave * 8), 0))")
+    w("#define ST(k, v) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (double)(v)), rs, vo, (k) * (kWave * 8), 0)")
     w("")
-
-    # ---------------- feval ----------------
-    w("// f(y) with y at workspace offset yoff (OFF_Y or OFF_YP)")
-    w("__device__ __noinline__ void feval(double *wb, int yoff)")
+    out += body
+    # cell load / store: inputs and results are cell-fastest global arrays
+    w("__device__ __noinline__ void load_cell(const SolveParams &P, int64_t cell, double *base_)")
     w("{")
-    for q, r in enumerate(day.active):
-        terms = [f"W(RK, {r})"] + [f"AT(yoff, {a})" for a in sym.reactants[r]]
-        w(f"    W(RATE, {r}) = {' * '.join(terms)};")
-        if q % FEVAL_RATE_GROUP == FEVAL_RATE_GROUP - 1:
-            w("    FENCE();")
-    w("    FENCE();")
-    contrib = [[] for _ in range(ns)]
-    for r in day.active:
-        for a in sym.reactants[r]:
-            contrib[a].append(f"a = a - W(RATE, {r});")
-        for k, p in enumerate(sym.products[r]):
-            c = sym.sc[r][k]
-            if c == 1.0:
-                contrib[p].append(f"a = a + W(RATE, {r});")
-            else:
-                contrib[p].append(f"a = a + {_lit(c)} * W(RATE, {r});")
-    for i in range(ns):
-        if contrib[i]:
-            w("    { double a = 0.0; " + " ".join(contrib[i]) + f" W(F, {i}) = a; }}")
-        else:
-            w(f"    W(F, {i}) = 0.0;")
-        if i % FEVAL_SPC_GROUP == FEVAL_SPC_GROUP - 1:
-            w("    FENCE();")
-    w("}")
-    w("")
-
-    # ---------------- jac_lu ----------------
-    def explic(r, ial):
-        re = sym.reactants[r]
-        if len(re) == 1:
-            return f"W(RK, {r})"
-        if len(re) == 2:
-            return f"(W(RK, {r}) * W(Y, {re[1 - ial]}))"
-        if ial == 0:
-            return f"(W(RK, {r}) * W(Y, {re[1]}) * W(Y, {re[2]}))"
-        if ial == 1:
-            return f"(W(RK, {r}) * W(Y, {re[0]}) * W(Y, {re[2]}))"
-        return f"(W(RK, {r}) * W(Y, {re[1]}) * W(Y, {re[0]}))"
-
-    jac_terms = [[] for _ in range(nnz)]
-    for r in day.active:
-        for slot, ial in day.jac_loss[r]:
-            jac_terms[slot].append(f"a = a - {explic(r, ial)};")
-        for slot, ial, iap in day.jac_prod[r]:
-            if iap < 0:
-                jac_terms[slot].append(f"a = a + {explic(r, ial)};")
-            else:
-                jac_terms[slot].append(f"a = a + {_lit(sym.sc[r][iap])} * {explic(r, ial)};")
-
-    col_slots = [[] for _ in range(ns)]
-    for s in range(nnz):
-        col_slots[day.cols[s]].append(s)
-    w("__device__ __noinline__ void jac_lu(double *wb, double gdtinv)")
-    w("{")
-    for j in range(ns):
-        w(f"    {{ // column {j}")
-        for s in col_slots[j]:
-            terms = jac_terms[s]
-            if terms:
-                body = " ".join(terms)
-                if s == day.diag[j]:
-                    body += " a = a - gdtinv;"
-                w(f"        double e{s}; {{ double a = 0.0; {body} e{s} = a; }}")
-            else:
-                if s == day.diag[j]:
-                    w(f"        double e{s} = 0.0 - gdtinv;")
-                else:
-                    w(f"        double e{s} = 0.0;")
-        beg, end = day.lu_col_ops[j]
-        for (ij, ik, kj) in day.lu_ops[beg:end]:
-            w(f"        e{ij} = e{ij} - W(LU, {ik}) * e{kj};")
-        d = day.diag[j]
-        w(f"        const double v = 1.0 / e{d}; W(VD, {j}) = v;")
-        for s in day.l_scale[j]:
-            w(f"        e{s} = e{s} * v;")
-        for s in col_slots[j]:
-            w(f"        W(LU, {s}) = e{s};")
-        w("    }")
-        w("    FENCE();")
-    w("}")
-    w("")
-
-    # ---------------- solve ----------------
-    w("// stage solve into x at workspace offset xoff;")
-    w("// mode 1: rhs = -f;  2: rhs = -f - x1*k1;  3: rhs = -f - x1*k1 - x2*k2")
-    w("__device__ __noinline__ void solve(double *wb, int xoff, double x1, double x2, int mode)")
-    w("{")
-    for i in range(ns):
-        terms = " ".join(f"r = r - W(LU, {s}) * AT(xoff, {c});" for (s, c) in day.lower[i])
-        w(f"    {{ double r = -W(F, {i}); if (mode >= 2) r = r - x1 * W(K1, {i}); "
-          f"if (mode == 3) r = r - x2 * W(K2, {i}); {terms} AT(xoff, {i}) = r; }}")
-        if i % SOLVE_ROW_GROUP == SOLVE_ROW_GROUP - 1:
-            w("    FENCE();")
-    for i in range(ns - 1, -1, -1):
-        terms = " ".join(f"r = r - W(LU, {s}) * AT(xoff, {c});" for (s, c) in day.upper[i])
-        w(f"    {{ double r = AT(xoff, {i}); {terms} AT(xoff, {i}) = r * W(VD, {i}); }}")
-        if i % SOLVE_ROW_GROUP == 0:
-            w("    FENCE();")
-    w("}")
-    w("")
-
-    # ---------------- cell load / store ----------------
-    w("__device__ __noinline__ void load_cell(const SolveParams &P, int64_t cell, double *wb)")
-    w("{")
+    w("    ROS3_PROLOGUE")
     w("    const int64_t n = P.ncells;")
-    w(f"    for (int r = 0; r < NR; ++r) W(RK, r) = P.rki[r * n + cell];")
+    w("    for (int r = 0; r < NR; ++r) ST(OFF_RK + r, P.rki[r * n + cell]);")
     for i in range(ns):
-        w(f"    W(Y, {i}) = P.y[{sym.inew2old[i]} * n + cell];")
+        w(f"    ST(OFF_Y + {i}, P.y[{sym.inew2old[i]} * n + cell]);")
     w("}")
-    w("__device__ __noinline__ void store_cell(const SolveParams &P, int64_t cell, const double *wb)")
+    w("__device__ __noinline__ void store_cell(const SolveParams &P, int64_t cell, double *base_)")
     w("{")
+    w("    ROS3_PROLOGUE")
     w("    const int64_t n = P.ncells;")
     for i in range(ns):
-        w(f"    P.y[{sym.inew2old[i]} * n + cell] = W(Y, {i});")
+        w(f"    P.y[{sym.inew2old[i]} * n + cell] = LD(OFF_Y + {i});")
     w("}")
     w(f"const int32_t INEW2OLD_HOST[NS] = {{{', '.join(str(v) for v in sym.inew2old)}}};")
     w("")
     w("#include \"../ros3_solver_kernel.inc\"")
-    w("#undef W")
-    w("#undef AT")
+    w("#undef LD")
+    w("#undef ST")
+    w("#undef ROS3_PROLOGUE")
     w(f"}} }} // namespace ros3::{cname}")
     w("")
-    # registration record consumed by the runtime
     w(f"extern \"C\" const ros3::MechKernels ros3_kernels_{cname} = {{")
     w(f"    \"{m['mechname']}\", ros3::{cname}::STRUCT_HASH, ros3::{cname}::NS, ros3::{cname}::NR,")
     w(f"    ros3::{cname}::NNZ, {len(day.lu_ops)}, {sym.night.nnz}, {len(sym.night.lu_ops)}, ros3::{cname}::NWS,")
-    w(f"    (const void *)&ros3::{cname}::solve_kernel, ros3::{cname}::INEW2OLD_HOST}};")
+    w(f"    (const void *)&ros3::{cname}::solve_kernel, ros3::{cname}::INEW2OLD_HOST,")
+    w(f"    \"{variant}\", {budget}, {lookahead}, ros3::{cname}::WAVES_PER_EU, {1 if is_default else 0}, {refill_min}}};")
     return "\n".join(out) + "\n", sym

@@ -1,0 +1,183 @@
+"""Per-step operation lists of the ROS3 kernel, in the reference's rounding order.
+
+Each builder returns a list of regalloc.Op.  The arithmetic of every slot is
+the sequence the reference applies to it:
+
+* feval   -- RBFEVAL (rbfeval.F:93-149): zero dc/dt, then reaction by
+             reaction: rate = RKI * y_1 [* y_2 [* y_3]]; each reactant loses
+             the rate, each product gains SC * rate;
+* jac_lu  -- RBJACOB (rbjacob.F:91-183) scatter order per entry, the diagonal
+             shift (rbsolver.F:377-381), RBDECOMP loop 1 (rbdecomp.F:86-102),
+             the pivot reciprocal (:105-108) and the L scaling (:115-136);
+             the op *schedule* across entries is free: "column" keeps the
+             reference's column order, "row" runs the same per-entry sequence
+             row by row (entry (i,j) still sees its updates in ascending k);
+* solve   -- RBSOLVE (rbsolve.F:78-187): each row sums its terms in storage
+             order, the backward pass ends with the multiply by VDIAG; the
+             stage right-hand side K1 = -F, K2 = -F - x1 K1, K3 = -F - x1 K1 -
+             x2 K2 (rbsolver.F:386-422) is formed in the row's first statement.
+"""
+
+from .regalloc import Op
+
+
+def _lit(x):
+    s = repr(float(x))
+    if "e" not in s and "." not in s and "inf" not in s and "nan" not in s:
+        s += ".0"
+    return s
+
+
+def feval_ops(sym, order="scatter"):
+    """f(y) with y read from array 'YS' into 'F'."""
+    day = sym.day
+    ops = []
+    if order == "scatter":
+        for i in range(sym.ns):
+            ops.append(Op("{0} = 0.0;", [("F", i)], False))
+        for r in day.active:
+            re = sym.reactants[r]
+            args = [("RATE", r), ("RK", r)] + [("YS", a) for a in re]
+            ops.append(Op("{0} = " + " * ".join("{%d}" % (k + 1) for k in range(len(re) + 1)) + ";",
+                          args, False))
+            for a in re:
+                ops.append(Op("{0} = {0} - {1};", [("F", a), ("RATE", r)], True))
+            for k, p in enumerate(sym.products[r]):
+                c = sym.sc[r][k]
+                if c == 1.0:
+                    ops.append(Op("{0} = {0} + {1};", [("F", p), ("RATE", r)], True))
+                else:
+                    ops.append(Op("{0} = {0} + " + _lit(c) + " * {1};", [("F", p), ("RATE", r)], True))
+        return ops
+    # gather: all rates, then one species at a time
+    for r in day.active:
+        re = sym.reactants[r]
+        args = [("RATE", r), ("RK", r)] + [("YS", a) for a in re]
+        ops.append(Op("{0} = " + " * ".join("{%d}" % (k + 1) for k in range(len(re) + 1)) + ";",
+                      args, False))
+    contrib = [[] for _ in range(sym.ns)]
+    for r in day.active:
+        for a in sym.reactants[r]:
+            contrib[a].append(("-", r, None))
+        for k, p in enumerate(sym.products[r]):
+            contrib[p].append(("+", r, sym.sc[r][k]))
+    for i in range(sym.ns):
+        ops.append(Op("{0} = 0.0;", [("F", i)], False))
+        for sign, r, c in contrib[i]:
+            if sign == "-":
+                ops.append(Op("{0} = {0} - {1};", [("F", i), ("RATE", r)], True))
+            elif c == 1.0:
+                ops.append(Op("{0} = {0} + {1};", [("F", i), ("RATE", r)], True))
+            else:
+                ops.append(Op("{0} = {0} + " + _lit(c) + " * {1};", [("F", i), ("RATE", r)], True))
+    return ops
+
+
+def _jac_terms(sym):
+    """Per LU slot, the Jacobian contributions in RBJACOB's order:
+    (sign, coefficient or None, reaction, reactant position)."""
+    day = sym.day
+    terms = [[] for _ in range(day.nnz)]
+    for r in day.active:
+        for slot, ial in day.jac_loss[r]:
+            terms[slot].append(("-", None, r, ial))
+        for slot, ial, iap in day.jac_prod[r]:
+            terms[slot].append(("+", None if iap < 0 else sym.sc[r][iap], r, ial))
+    return terms
+
+
+def _explic(sym, r, ial):
+    """Operand slots and expression of RBJACOB's EXPLIC(ial) (rbjacob.F:115-144)."""
+    re = sym.reactants[r]
+    rk = ("RK", r)
+    if len(re) == 1:
+        return [rk], "{%d}"
+    if len(re) == 2:
+        return [rk, ("Y", re[1 - ial])], "({%d} * {%d})"
+    if ial == 0:
+        return [rk, ("Y", re[1]), ("Y", re[2])], "({%d} * {%d} * {%d})"
+    if ial == 1:
+        return [rk, ("Y", re[0]), ("Y", re[2])], "({%d} * {%d} * {%d})"
+    return [rk, ("Y", re[1]), ("Y", re[0])], "({%d} * {%d} * {%d})"
+
+
+def _entry_ops(sym, terms, s, diag):
+    ops = []
+    if not terms[s]:
+        ops.append(Op("{0} = 0.0;", [("LU", s)], False))
+    else:
+        ops.append(Op("{0} = 0.0;", [("LU", s)], False))
+        for sign, c, r, ial in terms[s]:
+            srcs, ex = _explic(sym, r, ial)
+            ex = ex % tuple(range(1, len(srcs) + 1))
+            if sign == "-":
+                fmt = "{0} = {0} - " + ex + ";"
+            elif c is None:
+                fmt = "{0} = {0} + " + ex + ";"
+            else:
+                fmt = "{0} = {0} + " + _lit(c) + " * " + ex + ";"
+            ops.append(Op(fmt, [("LU", s)] + srcs, True))
+    if s == diag:
+        ops.append(Op("{0} = {0} - {1};", [("LU", s), "gdtinv"], True))
+    return ops
+
+
+def jac_lu_ops(sym, order="column"):
+    day = sym.day
+    n = sym.ns
+    terms = _jac_terms(sym)
+    ops = []
+    if order == "column":
+        col_slots = [[] for _ in range(n)]
+        for s in range(day.nnz):
+            col_slots[day.cols[s]].append(s)
+        for j in range(n):
+            for s in col_slots[j]:
+                ops += _entry_ops(sym, terms, s, day.diag[j])
+            beg, end = day.lu_col_ops[j]
+            for (ij, ik, kj) in day.lu_ops[beg:end]:
+                ops.append(Op("{0} = {0} - {1} * {2};", [("LU", ij), ("LU", ik), ("LU", kj)], True))
+            ops.append(Op("{0} = 1.0 / {1};", [("VD", j), ("LU", day.diag[j])], False))
+            for s in day.l_scale[j]:
+                ops.append(Op("{0} = {0} * {1};", [("LU", s), ("VD", j)], True))
+        return ops
+    # row order: row i's entries, then for k ascending: finish L(i,k), apply its updates
+    by_row = [dict() for _ in range(n)]       # i -> k -> [(ij, kj)]
+    for (ij, ik, kj) in day.lu_ops:
+        i, k = day.rows[ij], day.cols[ik]
+        by_row[i].setdefault(k, []).append((ij, ik, kj))
+    row_slots = [[] for _ in range(n)]
+    for s in range(day.nnz):
+        row_slots[day.rows[s]].append(s)
+    for i in range(n):
+        for s in row_slots[i]:
+            ops += _entry_ops(sym, terms, s, day.diag[i])
+        lks = sorted(day.cols[s] for s in row_slots[i] if day.cols[s] < i)
+        for k in lks:
+            s_ik = day.slot[(i, k)]
+            ops.append(Op("{0} = {0} * {1};", [("LU", s_ik), ("VD", k)], True))
+            for (ij, ik, kj) in by_row[i].get(k, []):
+                ops.append(Op("{0} = {0} - {1} * {2};", [("LU", ij), ("LU", ik), ("LU", kj)], True))
+        ops.append(Op("{0} = 1.0 / {1};", [("VD", i), ("LU", day.diag[i])], False))
+    return ops
+
+
+def solve_ops(sym, stage):
+    """Stage solve: x = K<stage>, right-hand side per rbsolver.F:386-422."""
+    day = sym.day
+    x = f"K{stage}"
+    ops = []
+    for i in range(sym.ns):
+        if stage == 1:
+            ops.append(Op("{0} = -{1};", [(x, i), ("F", i)], False))
+        elif stage == 2:
+            ops.append(Op("{0} = -{1} - x1 * {2};", [(x, i), ("F", i), ("K1", i)], False))
+        else:
+            ops.append(Op("{0} = -{1} - x1 * {2} - x2 * {3};", [(x, i), ("F", i), ("K1", i), ("K2", i)], False))
+        for (s, c) in day.lower[i]:
+            ops.append(Op("{0} = {0} - {1} * {2};", [(x, i), ("LU", s), (x, c)], True))
+    for i in range(sym.ns - 1, -1, -1):
+        for (s, c) in day.upper[i]:
+            ops.append(Op("{0} = {0} - {1} * {2};", [(x, i), ("LU", s), (x, c)], True))
+        ops.append(Op("{0} = {0} * {1};", [(x, i), ("VD", i)], True))
+    return ops

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -34,6 +35,7 @@ thread_local std::string g_last_error = "";
 struct cmaq_ros3_ctx {
     int device = 0;
     const MechKernels *mk = nullptr;
+    std::vector<const MechKernels *> variants;   // kernels built for this mechanism
     int ns = 0, nr = 0, nphotab = 0, nhetero = 0;
     std::vector<RateOp> ops;
     std::vector<RateMul> mul;
@@ -42,6 +44,7 @@ struct cmaq_ros3_ctx {
     int32_t refill_min = 0;
     int waves_per_cu = 0;
     int grid_blocks = 0;      // solver workgroups
+    int num_cus = 0;
     int64_t waves = 0;
     // device buffers
     RateOp *d_ops = nullptr;
@@ -293,6 +296,32 @@ int dalloc(cmaq_ros3_ctx *c, T **p, size_t n)
 
 }  // namespace
 
+// Grid and workspace for one kernel variant: as many resident workgroups as
+// the variant's occupancy allows (or the caller's waves-per-CU cap), one
+// lane-private workspace per resident lane.
+static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
+{
+    int per_cu = 0;
+    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mk->solve_kernel, kSolveBlock, 0));
+    if (per_cu < 1) per_cu = 1;
+    const int wpb = kSolveBlock / kWave;
+    if (c->waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, c->waves_per_cu / wpb));
+    const int grid = c->num_cus * per_cu;
+    const int64_t waves = (int64_t)grid * wpb;
+    const size_t need = (size_t)waves * mk->nws * kWave;
+    const size_t have = c->mk && c->d_ws ? (size_t)c->waves * c->mk->nws * kWave : 0;
+    if (need > have) {
+        hipFree(c->d_ws);
+        c->d_ws = nullptr;
+        if (hipMalloc((void **)&c->d_ws, need * sizeof(double)) != hipSuccess)
+            return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of the solver workspace failed");
+    }
+    c->mk = mk;
+    c->grid_blocks = grid;
+    c->waves = std::max<int64_t>(waves, have / ((size_t)mk->nws * kWave));
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -305,11 +334,18 @@ extern "C" int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech, c
     if (validate(mech, why) != 0) return set_err(nullptr, CMAQ_ROS3_E_MECH, why);
     Tab t{mech};
     const unsigned long long h = structure_hash(t);
-    const MechKernels *mk = nullptr;
+    std::vector<const MechKernels *> found;
     for (int i = 0; i < ros3_mech_registry_size; ++i)
         if (ros3_mech_registry[i]->struct_hash == h && ros3_mech_registry[i]->ns == mech->numb_mech_spc &&
             ros3_mech_registry[i]->nr == mech->nrxns)
-            mk = ros3_mech_registry[i];
+            found.push_back(ros3_mech_registry[i]);
+    const MechKernels *mk = nullptr;
+    const char *want = std::getenv("CMAQ_ROS3_VARIANT");
+    for (const MechKernels *k : found)
+        if (want && !std::strcmp(want, k->variant)) mk = k;
+    for (const MechKernels *k : found)
+        if (!mk && k->is_default) mk = k;
+    if (!mk && !found.empty()) mk = found[0];
     if (!mk) {
         char buf[160];
         std::snprintf(buf, sizeof buf, "no kernels built for this mechanism (structure hash %016llx)", h);
@@ -318,6 +354,7 @@ extern "C" int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech, c
     if (mech->nrxns > kMaxRates) return set_err(nullptr, CMAQ_ROS3_E_MECH, "more reactions than the rates kernel stages");
     cmaq_ros3_ctx *c = new cmaq_ros3_ctx();
     c->mk = mk;
+    c->variants = found;
     c->ns = mech->numb_mech_spc;
     c->nr = mech->nrxns;
     c->nphotab = mech->nphotab;
@@ -351,19 +388,13 @@ extern "C" int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech, c
         INIT_TRY(hipSetDevice(dev));
         hipDeviceProp_t prop;
         INIT_TRY(hipGetDeviceProperties(&prop, dev));
-        int per_cu = 0;
-        INIT_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)mk->solve_kernel, kSolveBlock, 0));
-        if (per_cu < 1) per_cu = 1;
-        const int wpb = kSolveBlock / kWave;
-        if (c->waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, c->waves_per_cu / wpb));
-        c->grid_blocks = prop.multiProcessorCount * per_cu;
-        c->waves = (int64_t)c->grid_blocks * wpb;
-        if (c->refill_min <= 0) c->refill_min = 16;
+        c->num_cus = prop.multiProcessorCount;
         if (c->refill_min > kWave) c->refill_min = kWave;
+        rc = configure(c, mk);
+        if (rc) goto fail;
         if (dalloc(c, &c->d_ops, c->ops.size())) goto fail_nomem;
         if (dalloc(c, &c->d_mul, std::max<size_t>(c->mul.size(), 1))) goto fail_nomem;
         if (dalloc(c, &c->d_rtol, c->ns) || dalloc(c, &c->d_atol, c->ns)) goto fail_nomem;
-        if (dalloc(c, &c->d_ws, (size_t)c->waves * mk->nws * kWave)) goto fail_nomem;
         if (dalloc(c, (char **)&c->d_small, 256)) goto fail_nomem;
         c->d_queue = (unsigned long long *)c->d_small;
         c->d_fail_cell = (int64_t *)((char *)c->d_small + 8);
@@ -418,6 +449,24 @@ extern "C" int cmaq_ros3_query(const cmaq_ros3_ctx *c, const char *key, int64_t 
     else if (!std::strcmp(key, "workspace_doubles_per_lane")) *value = k->nws;
     else return CMAQ_ROS3_E_ARG;
     return 0;
+}
+
+extern "C" int cmaq_ros3_set_variant(cmaq_ros3_ctx *c, const char *variant)
+{
+    if (!c || !variant) return CMAQ_ROS3_E_ARG;
+    for (const MechKernels *k : c->variants)
+        if (!std::strcmp(k->variant, variant)) {
+            HIP_TRY(c, hipSetDevice(c->device));
+            return configure(c, k);
+        }
+    return set_err(c, CMAQ_ROS3_E_ARG, std::string("no kernel variant ") + variant);
+}
+
+extern "C" const char *cmaq_ros3_variant(const cmaq_ros3_ctx *c, int index)
+{
+    if (!c) return nullptr;
+    if (index < 0) return c->mk->variant;
+    return index < (int)c->variants.size() ? c->variants[index]->variant : nullptr;
 }
 
 extern "C" int cmaq_ros3_last_timing(const cmaq_ros3_ctx *c, double *rates_ms, double *solve_ms)
@@ -497,7 +546,8 @@ extern "C" int cmaq_ros3_run_device(cmaq_ros3_ctx *c, int64_t n, double chemstep
     P.ncells = n; P.chemstep = chemstep; P.rki = c->d_rki; P.y = y; P.nacc = nacc; P.nrej = nrej;
     P.rtol = c->d_rtol; P.atol = c->d_atol; P.ws = c->d_ws; P.queue = c->d_queue;
     P.fail_flag = c->d_fail_flag; P.fail_cell = c->d_fail_cell; P.fail_info = c->d_fail_info;
-    P.fail_code = c->d_fail_code; P.max_attempts = c->max_attempts; P.refill_min = c->refill_min;
+    P.fail_code = c->d_fail_code; P.max_attempts = c->max_attempts;
+    P.refill_min = c->refill_min > 0 ? c->refill_min : (c->mk->refill_min > 0 ? c->mk->refill_min : 2);
     void *args[] = {&P};
     HIP_TRY(c, hipLaunchKernel(c->mk->solve_kernel, dim3(c->grid_blocks), dim3(kSolveBlock), args, 0, s));
     HIP_TRY(c, hipEventRecord(c->ev[2], s));

@@ -20,6 +20,7 @@
 namespace ros3 {
 
 constexpr int kWave = 64;
+typedef __attribute__((ext_vector_type(2))) unsigned int u32x2;
 
 // ---- ROS3 (Sandu et al. 1997, KPP) in the reference's G-form ----------
 constexpr double GAM = 0.43586652150845899941601945119356e+00;

@@ -12,6 +12,10 @@ struct MechKernels {
     int ns, nr, nnz_day, lu_ops_day, nnz_night, lu_ops_night, nws;
     const void *solve_kernel;         // host stub of solve_kernel
     const int32_t *inew2old;          // solver order -> mechanism order (0-based)
+    const char *variant;              // code-generation variant, e.g. "b48_l8"
+    int reg_budget, lookahead, waves_per_eu;
+    int is_default;                   // chosen when no variant is requested
+    int refill_min;                   // tuned refill threshold of this variant
 };
 
 }  // namespace ros3

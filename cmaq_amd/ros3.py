@@ -87,6 +87,9 @@ def lib():
         L.cmaq_ros3_last_error.restype = C.c_char_p
         L.cmaq_ros3_free.argtypes = [vp]
         L.cmaq_ros3_last_timing.argtypes = [vp, _f64p, _f64p]
+        L.cmaq_ros3_set_variant.argtypes = [vp, C.c_char_p]
+        L.cmaq_ros3_variant.argtypes = [vp, C.c_int]
+        L.cmaq_ros3_variant.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -187,6 +190,24 @@ class Ros3Solver:
             raise self._err(rc)
         return v.value
 
+    @property
+    def variant(self):
+        return lib().cmaq_ros3_variant(self._ctx, -1).decode()
+
+    def variants(self):
+        out, i = [], 0
+        while True:
+            v = lib().cmaq_ros3_variant(self._ctx, i)
+            if v is None:
+                return out
+            out.append(v.decode())
+            i += 1
+
+    def set_variant(self, name):
+        rc = lib().cmaq_ros3_set_variant(self._ctx, name.encode())
+        if rc != 0:
+            raise self._err(rc)
+
     def last_timing(self):
         """(rates_ms, solve_ms) of the last run, from HIP events on its stream."""
         a, b = C.c_double(), C.c_double()
@@ -198,8 +219,11 @@ class Ros3Solver:
         lib().cmaq_ros3_species_order(self._ctx, out.ctypes.data_as(_i32p))
         return out
 
-    def run(self, chemstep, temp, pres_atm, h2o_ppm, rj, y, het=None, seawater=None):
-        """Host arrays (numpy float64); returns (y_new, n_accepted, n_rejected)."""
+    def run(self, chemstep, temp, pres_atm, h2o_ppm, rj, y, het=None, seawater=None, raise_on_fail=True):
+        """Host arrays (numpy float64); returns (y_new, n_accepted, n_rejected).
+
+        With raise_on_fail=False a convergence failure still returns the
+        arrays (failed cells have n_accepted = -1)."""
         temp = np.ascontiguousarray(temp, np.float64)
         n = temp.size
         pres_atm = np.ascontiguousarray(pres_atm, np.float64)
@@ -214,7 +238,7 @@ class Ros3Solver:
         rc = lib().cmaq_ros3_run(self._ctx, n, chemstep, p(temp), p(pres_atm), p(h2o_ppm), p(rj),
                                  p(het_p), p(sw_p), p(yy), na.ctypes.data_as(_i32p), nj.ctypes.data_as(_i32p),
                                  C.byref(f))
-        if rc != 0:
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
             raise self._err(rc, {"cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow})
         return yy, na, nj
 

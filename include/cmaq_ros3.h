@@ -151,6 +151,16 @@ int cmaq_ros3_rates_device(cmaq_ros3_ctx *ctx, int64_t ncells,
  */
 int cmaq_ros3_last_timing(const cmaq_ros3_ctx *ctx, double *rates_ms, double *solve_ms);
 
+/*
+ * Kernel variants: every mechanism is generated in several register-budget /
+ * load-lookahead configurations (chosen per mechanism by the autotuner and
+ * overridable with the CMAQ_ROS3_VARIANT environment variable at init).
+ * cmaq_ros3_variant(ctx, -1) names the active one, index >= 0 lists them
+ * (NULL past the end); cmaq_ros3_set_variant switches.
+ */
+int cmaq_ros3_set_variant(cmaq_ros3_ctx *ctx, const char *variant);
+const char *cmaq_ros3_variant(const cmaq_ros3_ctx *ctx, int index);
+
 /* Sizes of the selected pattern: key in {"ischan","nnz_day","nnz_night",
  * "lu_ops_day","lu_ops_night","nrxns","numb_mech_spc","grid_waves"}. */
 int cmaq_ros3_query(const cmaq_ros3_ctx *ctx, const char *key, int64_t *value);

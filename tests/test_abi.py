@@ -1,0 +1,98 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+that include/cmaq_ros3.h declares, and cmaq_ros3_init's host-side validation
+(table checks, structure-hash lookup of the generated kernels) behaves as
+documented before any device is touched."""
+import copy
+import os
+import re
+
+import pytest
+
+from cmaq_amd import ros3
+from cmaq_amd.codegen.emit_hip import structure_hash
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "cmaq_ros3.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(cmaq_ros3_[a-z_]+)\s*\(", text)))
+
+
+def _lib():
+    if not os.path.exists(ros3.LIB_PATH):
+        pytest.skip("libcmaq_ros3.so not built (python -m cmaq_amd.build)")
+    return ros3.lib()
+
+
+def test_exports_every_declared_symbol():
+    L = _lib()
+    names = _declared()
+    assert len(names) >= 11
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_registry_holds_both_mechanisms():
+    """Every mechanism table shipped has generated kernels: the C++ lookup by
+    structure hash matches the generator's (init fails with E_MECH only when
+    no kernels carry the hash, and with E_HIP later when no device exists)."""
+    L = _lib()
+    for name in ("cb6r5_ae7_aq", "cracmm2"):
+        m = ros3.load_mechanism(name)
+        rc, msg = _init_rc(L, m)
+        assert rc in (0, -3), (name, rc, msg)   # ok, or no GPU in this process
+        assert "no kernels" not in msg
+
+
+def _init_rc(L, m):
+    import ctypes as C
+    s, keep = ros3.pack_mechanism(m)
+    ctx = C.c_void_p()
+    rc = L.cmaq_ros3_init(C.byref(ctx), C.byref(s), None, None, None)
+    msg = L.cmaq_ros3_last_error(None).decode()
+    if rc == 0:
+        L.cmaq_ros3_free(ctx)
+    del keep
+    return rc, msg
+
+
+def test_structural_change_is_rejected(cb6r5):
+    """A table whose structure differs from every generated kernel is refused
+    with E_MECH naming the hash the Python generator also computes."""
+    L = _lib()
+    m = copy.deepcopy(cb6r5)
+    m["sc"][5][0] = m["sc"][5][0] + 0.125
+    rc, msg = _init_rc(L, m)
+    assert rc == -2
+    assert f"{structure_hash(m):016x}" in msg
+
+
+@pytest.mark.parametrize("field,value,expect", [
+    ("numb_mech_spc", 0, "sizes"),
+    ("nspecial_rxn", 3, "SPECIAL"),
+])
+def test_invalid_tables(cb6r5, field, value, expect):
+    L = _lib()
+    m = copy.deepcopy(cb6r5)
+    m[field] = value
+    s, keep = ros3.pack_mechanism(m)
+    if field == "nspecial_rxn":
+        s.nspecial_rxn = value
+    import ctypes as C
+    ctx = C.c_void_p()
+    rc = L.cmaq_ros3_init(C.byref(ctx), C.byref(s), None, None, None)
+    msg = L.cmaq_ros3_last_error(None).decode()
+    assert rc in (-1, -2), (rc, msg)
+    assert expect.lower() in msg.lower(), msg
+
+
+def test_reserved_rate_types_rejected(cb6r5):
+    """KTYPE 7 / 11 / 13 (SPECIAL and Fortran-text rates) do not occur in the
+    shipped mechanisms and are refused rather than evaluated wrongly."""
+    L = _lib()
+    m = copy.deepcopy(cb6r5)
+    r = m["ktype"].index(3)
+    m["ktype"][r] = 13
+    rc, msg = _init_rc(L, m)
+    assert rc == -2 and "type" in msg.lower(), msg

@@ -70,3 +70,151 @@ def test_rates_match_oracle(cb6r5, oracle_cb6r5, solver_cb6r5):
 def test_config0_parity_cb6r5(cb6r5, oracle_cb6r5, solver_cb6r5):
     d = make_cells(cb6r5, np.arange(4096), seed=0)
     _compare(oracle_cb6r5, cb6r5, solver_cb6r5, d, "cb6r5 config0")
+
+
+@pytest.fixture(scope="module")
+def solver_cracmm2():
+    from cmaq_amd.ros3 import Ros3Solver
+    return Ros3Solver("cracmm2", RTOL, ATOL)
+
+
+def test_config0_parity_cracmm2(cracmm2, oracle_cracmm2, solver_cracmm2):
+    d = make_cells(cracmm2, np.arange(2048), seed=0)
+    _compare(oracle_cracmm2, cracmm2, solver_cracmm2, d, "cracmm2 config0")
+
+
+def test_cracmm2_mixed_svoc_parity(cracmm2, oracle_cracmm2, solver_cracmm2):
+    d = make_cells(cracmm2, np.arange(2048), seed=4, workload="mixed", svoc_ppb=(1e-5, 5.0))
+    _compare(oracle_cracmm2, cracmm2, solver_cracmm2, d, "cracmm2 mixed+svoc")
+
+
+def test_regional_sample_parity(cb6r5, oracle_cb6r5, solver_cb6r5):
+    """Cells drawn across all 35 layers of the 459x299x35 domain."""
+    ncol = 459 * 299
+    cells = np.arange(4096, dtype=np.int64) * 1173 % (ncol * 35)
+    d = make_cells(cb6r5, cells, seed=1, workload="regional", nlay_cells=ncol)
+    _compare(oracle_cb6r5, cb6r5, solver_cb6r5, d, "cb6r5 regional sample")
+
+
+def test_mixed_regimes_parity(cb6r5, oracle_cb6r5, solver_cb6r5):
+    d = make_cells(cb6r5, np.arange(4096), seed=2, workload="mixed")
+    _compare(oracle_cb6r5, cb6r5, solver_cb6r5, d, "cb6r5 mixed")
+
+
+def _het_inputs(mech, n, seed):
+    rng = np.random.default_rng(seed)
+    het = np.exp(rng.uniform(np.log(1e-6), np.log(1e-1), (mech["nhetero"], n)))
+    sea = np.where(rng.uniform(size=n) < 0.5, rng.uniform(0.0, 1.0, n), 0.0)
+    return np.ascontiguousarray(het), sea
+
+
+def test_het_and_halogen_parity(cb6r5, oracle_cb6r5, solver_cb6r5):
+    """Heterogeneous rates and the seawater-dependent HAL_Ozone rate
+    (RXNS_FUNC_MODULE.F90:357-363, 887-) through the full solve."""
+    import torch
+    n = 2048
+    d = make_cells(cb6r5, np.arange(n), seed=5)
+    het, sea = _het_inputs(cb6r5, n, 5)
+    ref = oracle_cb6r5.run(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], d["y"], RTOL, ATOL,
+                           het=het, seawater=sea)
+    y = _dev(d["y"]).clone()
+    na = torch.zeros(n, dtype=torch.int32, device="cuda")
+    solver_cb6r5.run_device(5.0, _dev(d["temp"]), _dev(d["pres"]), _dev(d["h2o"]), _dev(d["rj"]), y, na,
+                            het=_dev(het), seawater=_dev(sea), raise_on_fail=False)
+    yg, nag = y.cpu().numpy(), na.cpu().numpy()
+    ok = ref["nacc"] >= 0
+    assert np.array_equal(nag < 0, ~ok)
+    tol = 0.1 * (RTOL * np.abs(ref["y"]) + ATOL)
+    assert ((np.abs(yg - ref["y"]) > tol)[:, ok]).sum() == 0
+    assert (nag[ok] == ref["nacc"][ok]).mean() >= 0.999
+
+
+def test_host_entry_point_matches_device(cb6r5, solver_cb6r5):
+    """cmaq_ros3_run (host buffers, staged through the context) returns what
+    cmaq_ros3_run_device returns on the same inputs, bit for bit."""
+    import torch
+    n = 1000
+    d = make_cells(cb6r5, np.arange(n), seed=6)
+    yh, nah, njh = solver_cb6r5.run(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], d["y"], raise_on_fail=False)
+    y = _dev(d["y"]).clone()
+    na = torch.zeros(n, dtype=torch.int32, device="cuda")
+    nj = torch.zeros(n, dtype=torch.int32, device="cuda")
+    solver_cb6r5.run_device(5.0, _dev(d["temp"]), _dev(d["pres"]), _dev(d["h2o"]), _dev(d["rj"]), y, na, nj,
+                            raise_on_fail=False)
+    assert np.array_equal(y.cpu().numpy(), yh)
+    assert np.array_equal(na.cpu().numpy(), nah) and np.array_equal(nj.cpu().numpy(), njh)
+
+
+@pytest.mark.parametrize("n", [1, 63, 65, 777])
+def test_ragged_cell_counts(cb6r5, oracle_cb6r5, solver_cb6r5, n):
+    d = make_cells(cb6r5, np.arange(n) + 100, seed=7)
+    _compare(oracle_cb6r5, cb6r5, solver_cb6r5, d, f"cb6r5 n={n}")
+
+
+def test_zero_cells_is_a_no_op(solver_cb6r5):
+    import torch
+    e = torch.zeros(0, dtype=torch.float64, device="cuda")
+    assert solver_cb6r5.run_device(5.0, e, e, e, e, e) is None
+
+
+def test_step_limit_reports_cell(cb6r5, solver_cb6r5):
+    """max_attempts bounds the attempts per cell; exceeding it is reported as
+    E_STEPS with the cell, the other cells still finish."""
+    import torch
+    from cmaq_amd.ros3 import Ros3Solver
+    s = Ros3Solver(cb6r5, RTOL, ATOL, max_attempts=3)
+    n = 256
+    d = make_cells(cb6r5, np.arange(n), seed=8)
+    y = _dev(d["y"]).clone()
+    na = torch.zeros(n, dtype=torch.int32, device="cuda")
+    f = s.run_device(5.0, _dev(d["temp"]), _dev(d["pres"]), _dev(d["h2o"]), _dev(d["rj"]), y, na,
+                     raise_on_fail=False)
+    assert f is not None and f["code"] == -5 and 0 <= f["cell"] < n
+    assert (na.cpu().numpy() < 0).all()     # no cell integrates 5 min in 3 attempts
+
+
+def test_every_variant_matches(cb6r5, oracle_cb6r5):
+    """Each generated register-budget variant gives the same answer (the
+    allocation never changes the arithmetic) -- bit-identical across variants."""
+    import torch
+    from cmaq_amd.ros3 import Ros3Solver
+    s = Ros3Solver(cb6r5, RTOL, ATOL)
+    n = 1024
+    d = make_cells(cb6r5, np.arange(n), seed=9)
+    outs = {}
+    for v in s.variants():
+        s.set_variant(v)
+        y = _dev(d["y"]).clone()
+        na = torch.zeros(n, dtype=torch.int32, device="cuda")
+        s.run_device(5.0, _dev(d["temp"]), _dev(d["pres"]), _dev(d["h2o"]), _dev(d["rj"]), y, na,
+                     raise_on_fail=False)
+        outs[v] = (y.cpu().numpy(), na.cpu().numpy())
+    vs = list(outs)
+    assert len(vs) >= 2
+    for v in vs[1:]:
+        assert np.array_equal(outs[v][0], outs[vs[0]][0]), v
+        assert np.array_equal(outs[v][1], outs[vs[0]][1]), v
+
+
+def test_consecutive_sync_steps(cb6r5, oracle_cb6r5, solver_cb6r5):
+    """Three 5-min sync steps in a row, state carried on the device."""
+    import torch
+    n = 1024
+    d = make_cells(cb6r5, np.arange(n), seed=10, workload="mixed")
+    y = _dev(d["y"]).clone()
+    yr = d["y"].copy()
+    na = torch.zeros(n, dtype=torch.int32, device="cuda")
+    for _ in range(3):
+        ref = oracle_cb6r5.run(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], yr, RTOL, ATOL)
+        solver_cb6r5.run_device(5.0, _dev(d["temp"]), _dev(d["pres"]), _dev(d["h2o"]), _dev(d["rj"]), y, na,
+                                raise_on_fail=False)
+        yr = ref["y"]
+        ok = ref["nacc"] >= 0
+        nag = na.cpu().numpy()
+        assert np.array_equal(nag < 0, ~ok)
+        tol = 0.1 * (RTOL * np.abs(yr) + ATOL)
+        # compare against the oracle restarted from the GPU state, so that
+        # tolerance-level differences do not compound over steps
+        yg = y.cpu().numpy()
+        assert ((np.abs(yg - yr) > tol)[:, ok]).sum() == 0
+        yr = yg.copy()
