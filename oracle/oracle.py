@@ -75,6 +75,11 @@ def lib():
                                   _i32p, _i32p, C.POINTER(_Fail)]
         L.om_run.argtypes = [vp, C.c_int64, C.c_int, C.c_double] + [_f64p] * 7 + [_f64p, _f64p,
                                                                                _i32p, _i32p, C.POINTER(_Fail)]
+        L.om_steps_cell.argtypes = [vp, C.c_int, C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_int,
+                                    C.POINTER(C.c_int)]
+        L.om_replay_cell.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, C.c_int]
+        L.om_adjoint_cell.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, C.c_int, _f64p, _f64p]
+        L.om_adjoint_run.argtypes = [vp, C.c_int64, C.c_double] + [_f64p] * 7 + [_i32p]
         _lib = L
     return _lib
 
@@ -230,3 +235,61 @@ class Oracle:
                           C.byref(f))
         return {"rc": rc, "y": yy, "nacc": na, "nrej": nj,
                 "fail": {"cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}}
+
+    # ---- discrete adjoint with the accepted steps held fixed (one cell) ----
+    # y / lam arguments here are in MECHANISM order; the day pattern is used.
+    def _to_new(self, v):
+        return np.ascontiguousarray(np.asarray(v, np.float64)[self.inew2old() - 1])
+
+    def _to_old(self, v_new):
+        out = np.empty_like(v_new)
+        out[self.inew2old() - 1] = v_new
+        return out
+
+    def steps_cell(self, chemstep, rki, y, rtol=1e-3, atol=1e-7, hcap=1 << 20):
+        """Accepted step sizes of one cell's per-cell RBSOLVER run, and y_end."""
+        rk = np.ascontiguousarray(rki, np.float64)
+        yn = self._to_new(y)
+        rt = self._to_new(np.broadcast_to(rtol, (self.ns,)))
+        at = self._to_new(np.broadcast_to(atol, (self.ns,)))
+        h = np.zeros(hcap)
+        nh = C.c_int()
+        rc = lib().om_steps_cell(self._ctx, 1, chemstep, _p(rk, _f64p), _p(yn, _f64p), _p(rt, _f64p),
+                                 _p(at, _f64p), _p(h, _f64p), hcap, C.byref(nh))
+        if rc != 0:
+            raise RuntimeError(f"om_steps_cell failed ({rc})")
+        return h[:nh.value].copy(), self._to_old(yn)
+
+    def replay_cell(self, rki, y, h):
+        rk = np.ascontiguousarray(rki, np.float64)
+        yn = self._to_new(y)
+        h = np.ascontiguousarray(h, np.float64)
+        lib().om_replay_cell(self._ctx, 1, _p(rk, _f64p), _p(yn, _f64p), _p(h, _f64p), h.size)
+        return self._to_old(yn)
+
+    def adjoint_cell(self, rki, y0, h, lam):
+        """(dL/dy0, dL/drki) for dL/dy_end = lam."""
+        rk = np.ascontiguousarray(rki, np.float64)
+        y0n = self._to_new(y0)
+        ln = self._to_new(lam)
+        h = np.ascontiguousarray(h, np.float64)
+        gk = np.zeros(self.nr)
+        lib().om_adjoint_cell(self._ctx, 1, _p(rk, _f64p), _p(y0n, _f64p), _p(h, _f64p), h.size,
+                              _p(ln, _f64p), _p(gk, _f64p))
+        return self._to_old(ln), gk
+
+    def adjoint_run(self, chemstep, rki, y0, lam, rtol=1e-3, atol=1e-7):
+        """All cells: returns dict(lam=dL/dy0, gk=dL/drki, y=y_end, nacc)."""
+        rki = np.ascontiguousarray(rki, np.float64)
+        nc = rki.shape[1]
+        y0 = np.ascontiguousarray(y0, np.float64)
+        lam = np.array(lam, np.float64, copy=True, order="C")
+        rt = np.ascontiguousarray(np.broadcast_to(rtol, (self.ns,)), np.float64)
+        at = np.ascontiguousarray(np.broadcast_to(atol, (self.ns,)), np.float64)
+        gk = np.zeros((self.nr, nc))
+        yend = np.zeros_like(y0)
+        na = np.zeros(nc, np.int32)
+        rc = lib().om_adjoint_run(self._ctx, nc, chemstep, _p(rki, _f64p), _p(y0, _f64p), _p(rt, _f64p),
+                                  _p(at, _f64p), _p(lam, _f64p), _p(gk, _f64p), _p(yend, _f64p),
+                                  _p(na, _i32p))
+        return {"rc": rc, "lam": lam, "gk": gk, "y": yend, "nacc": na}

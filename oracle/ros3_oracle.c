@@ -754,3 +754,269 @@ int om_run(const om_ctx *c, int64_t ncells, int blksize, double chemstep,
     free(rtn); free(atn); free(bt); free(bp); free(bh); free(bs); free(brj); free(bhet); free(brk); free(by);
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* Per-cell step tape, fixed-step replay and discrete adjoint          */
+/* ------------------------------------------------------------------ */
+/* The reference path has no adjoint (its sensitivity option is DDM-3D,
+ * ddm3d/DDM3D_CHEM.F).  These functions hold the accepted-step sequence of
+ * om_rbsolver fixed and differentiate the resulting map y0 -> y_end (and
+ * rki -> y_end) exactly; the step-size controller is not differentiated.
+ * Pinned by central finite differences of om_replay_cell and by
+ * om_replay_cell reproducing om_rbsolver bit for bit.  One cell (nc = 1),
+ * arrays in the solver's (new) species order. */
+
+/* One ROS3 step of size dt from y, as in om_rbsolver (rbsolver.F:357-430):
+ * cc2/vdiag = LU of J(y) - I/(GAM dt); k1..k3 the stages; yp = y + K1;
+ * yn = y + B1 K1 + B2 K2 + B3 K3 (before the CONMIN clamp). */
+static void ros3_stages1(const om_ctx *c, int ncsp, const double *rki, const double *y, double dt,
+                         double *ydot, double *cc2, double *vdiag, double *k1, double *k2,
+                         double *k3, double *yp, double *yn)
+{
+    const int ns = c->ns, nc = 1;
+    const double RGAM = 1.0 / GAM;
+    const int idiagbeg = c->iarray[ncsp] - c->ischan + 1;
+    om_feval(c, ncsp, nc, rki, y, ydot);
+    double dtinv = 1.0 / dt;
+    double gdtinv = dtinv * RGAM;
+    om_jacob(c, ncsp, nc, rki, y, cc2);
+    for (int j = idiagbeg; j <= c->iarray[ncsp]; ++j) cc2[j] = cc2[j] - gdtinv;
+    om_decomp(c, ncsp, nc, cc2, vdiag);
+    for (int q = 0; q < ns; ++q) k1[q] = -ydot[q];
+    om_solve(c, ncsp, nc, cc2, vdiag, k1);
+    for (int q = 0; q < ns; ++q) yp[q] = y[q] + k1[q];
+    om_feval(c, ncsp, nc, rki, yp, ydot);
+    double x1 = C21 * dtinv;
+    for (int q = 0; q < ns; ++q) k2[q] = -ydot[q] - x1 * k1[q];
+    om_solve(c, ncsp, nc, cc2, vdiag, k2);
+    x1 = C31 * dtinv;
+    double x2 = C32 * dtinv;
+    for (int q = 0; q < ns; ++q) k3[q] = -ydot[q] - x1 * k1[q] - x2 * k2[q];
+    om_solve(c, ncsp, nc, cc2, vdiag, k3);
+    for (int q = 0; q < ns; ++q) yn[q] = y[q] + B1 * k1[q] + B2 * k2[q] + B3 * k3[q];
+}
+
+typedef struct { double *ydot, *cc2, *vdiag, *k1, *k2, *k3, *yp, *yn, *u1, *u2, *u3, *yb, *s2, *s3; } om_work;
+
+static void work_alloc(const om_ctx *c, int ncsp, om_work *w)
+{
+    size_t ns = c->ns;
+    w->ydot = xcalloc(ns, 8); w->k1 = xcalloc(ns, 8); w->k2 = xcalloc(ns, 8); w->k3 = xcalloc(ns, 8);
+    w->yp = xcalloc(ns, 8); w->yn = xcalloc(ns, 8); w->u1 = xcalloc(ns, 8); w->u2 = xcalloc(ns, 8);
+    w->u3 = xcalloc(ns, 8); w->yb = xcalloc(ns, 8); w->vdiag = xcalloc(ns, 8);
+    w->cc2 = xcalloc((size_t)c->iarray[ncsp] + 1, 8);
+    w->s2 = xcalloc((size_t)c->nr + 1, 8); w->s3 = xcalloc((size_t)c->nr + 1, 8);
+}
+
+static void work_free(om_work *w)
+{
+    free(w->ydot); free(w->k1); free(w->k2); free(w->k3); free(w->yp); free(w->yn); free(w->u1);
+    free(w->u2); free(w->u3); free(w->yb); free(w->vdiag); free(w->cc2); free(w->s2); free(w->s3);
+}
+
+int om_steps_cell(const om_ctx *c, int ncsp, double chemstep, const double *rki, double *y,
+                  const double *rtol, const double *atol, double *h, int hcap, int *nh)
+{
+    /* om_rbsolver's controller on one cell, recording each accepted DT */
+    const int ns = c->ns;
+    const double rnspec = (double)(1.0f / (float)ns);
+    om_work w;
+    work_alloc(c, ncsp, &w);
+    double dt = DTMIN > DTSTART ? DTMIN : DTSTART, tnow = 0.0;
+    int lpass = 0, rc = 0;
+    long attempts = 0;
+    *nh = 0;
+    while (tnow < chemstep) {
+        if (++attempts > OM_MAX_ATTEMPTS) { rc = -2; break; }
+        double tend = tnow + dt;
+        if (tend > chemstep) { dt = chemstep - tnow; tend = chemstep; }
+        ros3_stages1(c, ncsp, rki, y, dt, w.ydot, w.cc2, w.vdiag, w.k1, w.k2, w.k3, w.yp, w.yn);
+        double err = 0.0;
+        for (int n = 0; n < ns; ++n) {
+            double ytol = atol[n] + rtol[n] * fabs(w.yn[n]);
+            double e = (D1 * w.k1[n] + D2 * w.k2[n] + D3 * w.k3[n]) / ytol;
+            err = err + e * e;
+        }
+        double s = sqrt(err * rnspec), maxerr = UROUND;
+        if (!(s <= maxerr)) maxerr = s;
+        double dtfac = 0.9 / pow(maxerr, GROW);
+        if (maxerr <= 1.0) {
+            if (*nh >= hcap) { rc = -3; break; }
+            h[(*nh)++] = dt;
+            for (int q = 0; q < ns; ++q) y[q] = w.yn[q] > CONMIN ? w.yn[q] : CONMIN;
+            tnow = tend;
+            double hi = lpass ? FACMAX : FACONE;
+            if (dtfac > hi) dtfac = hi;
+            if (dtfac < FACMIN) dtfac = FACMIN;
+            double dn = dtfac * dt;
+            if (dn < DTMIN) dn = DTMIN;
+            dt = dn < DTMAX ? dn : DTMAX;
+            lpass = 1;
+        } else {
+            dt = FACMIN * dt;
+            lpass = 0;
+            if (dt < DTMIN) { rc = -1; break; }
+        }
+    }
+    work_free(&w);
+    return rc;
+}
+
+void om_replay_cell(const om_ctx *c, int ncsp, const double *rki, double *y, const double *h, int nh)
+{
+    om_work w;
+    work_alloc(c, ncsp, &w);
+    for (int n = 0; n < nh; ++n) {
+        ros3_stages1(c, ncsp, rki, y, h[n], w.ydot, w.cc2, w.vdiag, w.k1, w.k2, w.k3, w.yp, w.yn);
+        for (int q = 0; q < c->ns; ++q) y[q] = w.yn[q] > CONMIN ? w.yn[q] : CONMIN;
+    }
+    work_free(&w);
+}
+
+/* Solves (L U)^T u = v in place with om_decomp's factors: U^T z = v by rows
+ * ascending, then L^T u = z by rows descending (scatter form). */
+static void tsolve1(const om_ctx *c, int ncsp, const double *cc2, const double *vdiag, double *v)
+{
+    const int N = c->ischan;
+    const int *kz = c->kzero[ncsp];
+    int ltot = 0;
+    for (int i = 1; i <= N; ++i) ltot += c->kzilch[ncsp][i];
+    /* U rows are stored N..1 after the L block */
+    int *ubeg = xcalloc((size_t)N + 2, sizeof(int)), *lbeg = xcalloc((size_t)N + 2, sizeof(int));
+    int q = ltot + 1;
+    for (int i = N; i >= 1; --i) { ubeg[i] = q; q += c->mzilch[ncsp][i]; }
+    q = 1;
+    for (int i = 1; i <= N; ++i) { lbeg[i] = q; q += c->kzilch[ncsp][i]; }
+    for (int i = 1; i <= N; ++i) {
+        v[i - 1] = v[i - 1] * vdiag[i - 1];
+        for (int e = ubeg[i]; e < ubeg[i] + c->mzilch[ncsp][i]; ++e)
+            v[kz[e] - 1] = v[kz[e] - 1] - cc2[e] * v[i - 1];
+    }
+    for (int i = N; i >= 1; --i)
+        for (int e = lbeg[i]; e < lbeg[i] + c->kzilch[ncsp][i]; ++e)
+            v[kz[e] - 1] = v[kz[e] - 1] - cc2[e] * v[i - 1];
+    free(ubeg); free(lbeg);
+}
+
+/* (S^T w)_r: -w over reactant slots, +SC w over product slots (rbfeval.F:120-147) */
+static double st_dot(const om_ctx *c, int r, const double *w)
+{
+    double s = 0.0;
+    for (int q = 1; q <= c->t.nreact[r - 1]; ++q) s -= w[IRM2(c, r, q) - 1];
+    for (int p = 1; p <= c->t.nprdct[r - 1]; ++p) s += SC(c, r, p) * w[IRM2(c, r, p + 3) - 1];
+    return s;
+}
+
+/* product of reactant concentrations except slots a and b (1-based; 0 = none) */
+static double pexcept(const om_ctx *c, int r, const double *y, int a, int b)
+{
+    double p = 1.0;
+    for (int q = 1; q <= c->t.nreact[r - 1]; ++q)
+        if (q != a && q != b) p *= y[IRM2(c, r, q) - 1];
+    return p;
+}
+
+/* Reverse of one step: lam holds dL/dy_{n+1} on entry, dL/dy_n on exit;
+ * gk[r-1] += dL/dk_r.  y is the step's initial state, dt its size. */
+static void adj_step1(const om_ctx *c, int ncsp, const double *rki, const double *y, double dt,
+                      double *lam, double *gk, om_work *w)
+{
+    const int ns = c->ns;
+    ros3_stages1(c, ncsp, rki, y, dt, w->ydot, w->cc2, w->vdiag, w->k1, w->k2, w->k3, w->yp, w->yn);
+    const double dtinv = 1.0 / dt;
+    const double x21 = C21 * dtinv, x31 = C31 * dtinv, x32 = C32 * dtinv;
+    for (int i = 0; i < ns; ++i) if (!(w->yn[i] > CONMIN)) lam[i] = 0.0;  /* clamp (rbsolver.F:473) */
+    for (int i = 0; i < ns; ++i) w->u3[i] = B3 * lam[i];
+    tsolve1(c, ncsp, w->cc2, w->vdiag, w->u3);
+    for (int i = 0; i < ns; ++i) w->u2[i] = B2 * lam[i] - x32 * w->u3[i];
+    tsolve1(c, ncsp, w->cc2, w->vdiag, w->u2);
+    /* F2 = f(YP) with F2bar = -(u2 + u3): YPbar = J(YP)^T F2bar */
+    for (int i = 0; i < ns; ++i) w->yb[i] = 0.0;
+    for (int nrx = 1; nrx <= c->nuserat[ncsp]; ++nrx) {
+        int r = c->nkuserat[ncsp][nrx];
+        w->s2[r] = st_dot(c, r, w->u2);
+        w->s3[r] = st_dot(c, r, w->u3);
+        double g = -rki[r - 1] * (w->s2[r] + w->s3[r]);
+        for (int q = 1; q <= c->t.nreact[r - 1]; ++q)
+            w->yb[IRM2(c, r, q) - 1] += g * pexcept(c, r, w->yp, q, 0);
+    }
+    for (int i = 0; i < ns; ++i) w->u1[i] = B1 * lam[i] - x31 * w->u3[i] - x21 * w->u2[i] + w->yb[i];
+    tsolve1(c, ncsp, w->cc2, w->vdiag, w->u1);
+    /* ybar = lam + YPbar + J(y)^T F1bar - sum_s H(u_s, K_s), F1bar = -u1 */
+    for (int i = 0; i < ns; ++i) w->yb[i] += lam[i];
+    const double *K[3] = {w->k1, w->k2, w->k3};
+    for (int nrx = 1; nrx <= c->nuserat[ncsp]; ++nrx) {
+        int r = c->nkuserat[ncsp][nrx], nre = c->t.nreact[r - 1];
+        double s1 = st_dot(c, r, w->u1), s2 = w->s2[r], s3 = w->s3[r];
+        double sig[3] = {s1, s2, s3};
+        double k = rki[r - 1];
+        double w1 = -s1, w2 = -(s2 + s3);
+        double dk = w1 * pexcept(c, r, y, 0, 0) + w2 * pexcept(c, r, w->yp, 0, 0);
+        for (int q = 1; q <= nre; ++q) {
+            int a = IRM2(c, r, q) - 1;
+            double gy = pexcept(c, r, y, q, 0);
+            double hs = 0.0;
+            for (int s = 0; s < 3; ++s) {
+                double hk = 0.0;   /* sum over the other reactant slots of K * remaining product */
+                for (int q2 = 1; q2 <= nre; ++q2)
+                    if (q2 != q) hk += K[s][IRM2(c, r, q2) - 1] * pexcept(c, r, y, q, q2);
+                hs += sig[s] * hk;
+                dk -= sig[s] * gy * K[s][a];
+            }
+            w->yb[a] += k * (w1 * gy - hs);
+        }
+        gk[r - 1] += dk;
+    }
+    for (int i = 0; i < ns; ++i) lam[i] = w->yb[i];
+}
+
+void om_adjoint_cell(const om_ctx *c, int ncsp, const double *rki, const double *y0,
+                     const double *h, int nh, double *lam, double *gk)
+{
+    const int ns = c->ns;
+    double *tape = xcalloc((size_t)(nh ? nh : 1) * ns, 8);
+    om_work w;
+    work_alloc(c, ncsp, &w);
+    double *y = xcalloc(ns, 8);
+    memcpy(y, y0, (size_t)ns * 8);
+    for (int n = 0; n < nh; ++n) {
+        memcpy(tape + (size_t)n * ns, y, (size_t)ns * 8);
+        ros3_stages1(c, ncsp, rki, y, h[n], w.ydot, w.cc2, w.vdiag, w.k1, w.k2, w.k3, w.yp, w.yn);
+        for (int q = 0; q < ns; ++q) y[q] = w.yn[q] > CONMIN ? w.yn[q] : CONMIN;
+    }
+    for (int n = nh - 1; n >= 0; --n) adj_step1(c, ncsp, rki, tape + (size_t)n * ns, h[n], lam, gk, &w);
+    work_free(&w);
+    free(y); free(tape);
+}
+
+int om_adjoint_run(const om_ctx *c, int64_t ncells, double chemstep, const double *rki,
+                   const double *y0, const double *rtol, const double *atol,
+                   double *lam, double *gk, double *yend, int32_t *nacc)
+{
+    const int ns = c->ns, nr = c->nr, hcap = 1 << 20;
+    double *h = xcalloc(hcap, 8), *y = xcalloc(ns, 8), *ys = xcalloc(ns, 8), *l = xcalloc(ns, 8);
+    double *rk = xcalloc(nr, 8), *g = xcalloc(nr, 8), *rt = xcalloc(ns, 8), *at = xcalloc(ns, 8);
+    for (int i = 1; i <= ns; ++i) { rt[i - 1] = rtol[c->inew2old[i] - 1]; at[i - 1] = atol[c->inew2old[i] - 1]; }
+    int rc = 0;
+    for (int64_t cc = 0; cc < ncells; ++cc) {
+        for (int r = 0; r < nr; ++r) { rk[r] = rki[(size_t)r * ncells + cc]; g[r] = 0.0; }
+        for (int i = 1; i <= ns; ++i) {
+            size_t o = (size_t)(c->inew2old[i] - 1) * ncells + cc;
+            y[i - 1] = ys[i - 1] = y0[o];
+            l[i - 1] = lam[o];
+        }
+        int nh = 0;
+        int brc = om_steps_cell(c, 1, chemstep, rk, y, rt, at, h, hcap, &nh);
+        if (nacc) nacc[cc] = brc ? -1 : nh;
+        if (brc) { if (!rc) rc = brc; continue; }
+        om_adjoint_cell(c, 1, rk, ys, h, nh, l, g);
+        for (int i = 1; i <= ns; ++i) {
+            size_t o = (size_t)(c->inew2old[i] - 1) * ncells + cc;
+            lam[o] = l[i - 1];
+            if (yend) yend[o] = y[i - 1];
+        }
+        if (gk) for (int r = 0; r < nr; ++r) gk[(size_t)r * ncells + cc] += g[r];
+    }
+    free(h); free(y); free(ys); free(l); free(rk); free(g); free(rt); free(at);
+    return rc;
+}

@@ -97,6 +97,25 @@ int om_run(const om_ctx *c, int64_t ncells, int blksize, double chemstep,
            double *y, const double *rtol, const double *atol,
            int32_t *nacc, int32_t *nrej, om_fail *fail);
 
+/* Discrete adjoint of the per-cell integration with its accepted-step
+ * sequence held fixed (no reference counterpart; see ros3_oracle.c).
+ * One cell, y / lam in NEW species order, rki [nr].
+ * om_steps_cell: per-cell RBSOLVER recording the accepted step sizes.
+ * om_replay_cell: the same map with the given steps and no controller.
+ * om_adjoint_cell: lam = dL/dy_end on entry, dL/dy0 on exit; gk += dL/drki. */
+int om_steps_cell(const om_ctx *c, int ncsp, double chemstep, const double *rki, double *y,
+                  const double *rtol, const double *atol, double *h, int hcap, int *nh);
+void om_replay_cell(const om_ctx *c, int ncsp, const double *rki, double *y, const double *h, int nh);
+void om_adjoint_cell(const om_ctx *c, int ncsp, const double *rki, const double *y0,
+                     const double *h, int nh, double *lam, double *gk);
+/* All cells (day pattern, per-cell control) with the rate constants given:
+ * rki [nr][ncells]; y0, lam, yend [ns][ncells] in MECHANISM order;
+ * gk [nr][ncells] accumulated; nacc = accepted steps or -1 (failed cell,
+ * lam left unchanged). */
+int om_adjoint_run(const om_ctx *c, int64_t ncells, double chemstep, const double *rki,
+                   const double *y0, const double *rtol, const double *atol,
+                   double *lam, double *gk, double *yend, int32_t *nacc);
+
 #ifdef __cplusplus
 }
 #endif

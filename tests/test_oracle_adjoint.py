@@ -1,0 +1,120 @@
+"""The oracle's discrete adjoint (accepted-step sequence held fixed).
+
+Pinned by: the fixed-step replay reproducing RBSOLVER bit for bit; central
+finite differences (h = 1e-6 relative) of the replayed map for dL/dy0 and
+dL/drki; the closed form of the linear decay A -> B (the adjoint of a linear
+map is its transpose: dL/dy0 of L = y_end(A) equals the product of the
+per-step amplification factors); and linearity in the seed vector.
+"""
+import numpy as np
+import pytest
+
+from cmaq_amd.inputs import make_cells
+
+FD_REL = 1.0e-5          # bar for adjoint vs central FD, relative to the largest entry
+
+
+def _cell(mech, o, seed, workload="clean"):
+    d = make_cells(mech, np.arange(1), seed=seed, workload=workload)
+    rki, _ = o.rates(d["temp"], d["pres"], d["h2o"], d["rj"])
+    return rki[:, 0].copy(), d["y"][:, 0].copy()
+
+
+def _seed(mech):
+    names = mech["chemistry_spc"]
+    w = np.zeros(len(names))
+    w[names.index("O3")] = 1.0
+    w[names.index("NO2")] = 0.1
+    return w
+
+
+def test_replay_reproduces_rbsolver(cb6r5, oracle_cb6r5):
+    o = oracle_cb6r5
+    for seed in (1, 2, 3):
+        rki, y0 = _cell(cb6r5, o, seed)
+        h, yend = o.steps_cell(5.0, rki, y0)
+        assert np.array_equal(o.replay_cell(rki, y0, h), yend)
+        inew = o.inew2old() - 1
+        ns = cb6r5["numb_mech_spc"]
+        rc, yb, na, _ = o.rbsolver(1, 5.0, rki[:, None], y0[inew][:, None], np.full(ns, 1e-3), np.full(ns, 1e-7))
+        assert rc == 0 and na == len(h) and np.array_equal(yb[:, 0], yend[inew])
+
+
+@pytest.mark.parametrize("seed,workload", [(3, "clean"), (7, "mixed"), (11, "mixed")])
+def test_adjoint_vs_central_fd(cb6r5, oracle_cb6r5, seed, workload):
+    o = oracle_cb6r5
+    rki, y0 = _cell(cb6r5, o, seed, workload)
+    h, _ = o.steps_cell(5.0, rki, y0)
+    w = _seed(cb6r5)
+    lam0, gk = o.adjoint_cell(rki, y0, h, w)
+
+    def L(y, k):
+        return w @ o.replay_cell(k, y, h)
+
+    # d/d ln y0 and d/d ln k: the semi-normalised forms compared in DDM-3D
+    sy, sk = lam0 * y0, gk * rki
+    for j in np.argsort(-np.abs(sy))[:12]:
+        e = 1e-6 * y0[j]
+        yp, ym = y0.copy(), y0.copy()
+        yp[j] += e
+        ym[j] -= e
+        fd = (L(yp, rki) - L(ym, rki)) / (2 * e) * y0[j]
+        assert abs(sy[j] - fd) <= FD_REL * np.abs(sy).max(), (j, sy[j], fd)
+    for r in np.argsort(-np.abs(sk))[:12]:
+        e = 1e-6 * rki[r]
+        kp, km = rki.copy(), rki.copy()
+        kp[r] += e
+        km[r] -= e
+        fd = (L(y0, kp) - L(y0, km)) / (2 * e) * rki[r]
+        assert abs(sk[r] - fd) <= FD_REL * np.abs(sk).max(), (r, sk[r], fd)
+
+
+def test_adjoint_linear_decay_closed_form():
+    """A -> B with rate k: each accepted step multiplies A by R(z), z = -k h,
+    R the ROS3 stability function, so dA_end/dA0 = prod R(z_n), and
+    dA_end/dk = A0 * d/dk prod R(-k h_n)."""
+    from oracle.oracle import Oracle
+    from test_oracle import _decay_mech
+    k = 0.7
+    o = Oracle(_decay_mech(k))
+    y0 = np.array([1.0, 0.0])
+    rki = np.array([k])
+    h, yend = o.steps_cell(5.0, rki, y0)
+    lam0, gk = o.adjoint_cell(rki, y0, h, np.array([1.0, 0.0]))
+    amp = yend[0] / y0[0]
+    assert abs(lam0[0] - amp) <= 1e-13 * abs(amp)
+    assert lam0[1] == 0.0
+    e = 1e-6 * k
+    fd = (o.replay_cell(np.array([k + e]), y0, h)[0] - o.replay_cell(np.array([k - e]), y0, h)[0]) / (2 * e)
+    assert abs(gk[0] - fd) <= 1e-6 * abs(fd)
+
+
+def test_adjoint_is_linear_in_seed(cb6r5, oracle_cb6r5):
+    o = oracle_cb6r5
+    rki, y0 = _cell(cb6r5, o, 5)
+    h, _ = o.steps_cell(5.0, rki, y0)
+    rng = np.random.default_rng(0)
+    a, b = rng.normal(size=y0.size), rng.normal(size=y0.size)
+    la, ga = o.adjoint_cell(rki, y0, h, a)
+    lb, gb = o.adjoint_cell(rki, y0, h, b)
+    lc, gc = o.adjoint_cell(rki, y0, h, 2.0 * a - 3.0 * b)
+    np.testing.assert_allclose(lc, 2.0 * la - 3.0 * lb, rtol=1e-9, atol=1e-12 * np.abs(lc).max())
+    np.testing.assert_allclose(gc, 2.0 * ga - 3.0 * gb, rtol=1e-9, atol=1e-12 * np.abs(gc).max())
+
+
+def test_adjoint_run_matches_cellwise(cb6r5, oracle_cb6r5):
+    o = oracle_cb6r5
+    n = 6
+    d = make_cells(cb6r5, np.arange(n), seed=12, workload="mixed")
+    rki = np.stack([o.rates(d["temp"][c:c + 1], d["pres"][c:c + 1], d["h2o"][c:c + 1],
+                            d["rj"][:, c:c + 1])[0][:, 0] for c in range(n)], axis=1)
+    w = _seed(cb6r5)
+    lam = np.repeat(w[:, None], n, axis=1)
+    r = o.adjoint_run(5.0, rki, d["y"], lam)
+    assert r["rc"] == 0
+    for c in range(n):
+        h, yend = o.steps_cell(5.0, rki[:, c], d["y"][:, c])
+        l0, g = o.adjoint_cell(rki[:, c], d["y"][:, c], h, w)
+        assert r["nacc"][c] == len(h)
+        assert np.array_equal(r["y"][:, c], yend)
+        assert np.array_equal(r["lam"][:, c], l0) and np.array_equal(r["gk"][:, c], g)
