@@ -4,7 +4,7 @@
 
 A "step" is one 5-minute chemistry sync step (CALC_RCONST + RBSOLVER) over
 every cell of the workload, with inputs already resident in HBM.  The
-default workload is the regional CB6r5 case: 459 x 299 x 35 = 4,803,315
+default workload is the regional CB6r5 case: 459 x 299 x 35 = 4,803,435
 cells, layer-major, J scaled by exp(-k/20) per layer.  The metric is
 attempted ROS3 cell-steps per second (accepted steps are reported too).
 
@@ -15,7 +15,12 @@ The JSON line also carries
                  output of one sync step) over its HIP-event-timed duration;
   cpu_baseline - the CPU oracle (a C restatement of the reference ROS3, built
                  with the reference's gcc flags) timed on a bounded sample of
-                 the same cells on the host cores, in stock 50-cell blocks.
+                 the same cells on the host cores, in stock 50-cell blocks;
+  fwd_adj_ratio - forward + discrete adjoint over forward alone, on the
+                 differentiation workload: 262,144 mixed-regime cells, 12
+                 sync steps of 5 min, loss = sum(O3 + 0.1 NO2) at the end,
+                 gradients with respect to all initial concentrations and
+                 all rate constants (details under "adjoint").
 """
 
 import argparse
@@ -42,6 +47,9 @@ def _args():
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline")
     p.add_argument("--cpu-workers", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--adj-cells", type=int, default=262144)
+    p.add_argument("--adj-sync-steps", type=int, default=12)
+    p.add_argument("--no-adj", action="store_true")
     return p.parse_args()
 
 
@@ -88,6 +96,61 @@ def cpu_baseline(args, mech, ncells, nlay_cells):
                       f"one 5-min step, 50-cell blocks (stock BLKSIZE, no stiffness pre-sort), "
                       f"{workers} processes on disjoint cells; slowest process {busy:.1f} s, wall {wall:.1f} s",
             "failed_blocks": int(sum(1 for r in res if r[3] != 0))}
+
+
+def adjoint_leg(args, mech, solver, dev):
+    """Times 12 sync steps forward, then forward + adjoint through autograd."""
+    import torch
+    from cmaq_amd.adjoint import integrate
+    from cmaq_amd.inputs import make_cells_torch
+    n, k = args.adj_cells, args.adj_sync_steps
+    d = make_cells_torch(mech, n, seed=args.seed + 3, workload="mixed", device=dev)
+    rki = torch.empty((mech["nrxns"], n), dtype=torch.float64, device=dev)
+    solver.rates_device(d["temp"], d["pres"], d["h2o"], d["rj"], rki)
+    names = mech["chemistry_spc"]
+    io3, ino2 = names.index("O3"), names.index("NO2")
+    y = torch.empty_like(d["y"])
+    na = torch.empty(n, dtype=torch.int32, device=dev)
+
+    def fwd():
+        y.copy_(d["y"])
+        att = 0
+        for _ in range(k):
+            solver.solve_device(5.0, rki, y, na, raise_on_fail=False)
+            att += int(na.clamp(min=0).sum())
+        return att
+
+    def fwd_adj():
+        y0 = d["y"].clone().requires_grad_(True)
+        kk = rki.clone().requires_grad_(True)
+        yend = integrate(solver, y0, kk, 5.0, k)
+        loss = (yend[io3] + 0.1 * yend[ino2]).sum()
+        loss.backward()
+        return y0.grad, kk.grad
+
+    fwd()                                   # warm-up (allocations, code load)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    accepted = fwd()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    print(f"[bench] adjoint leg: forward {t_fwd * 1e3:.0f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gy, gk = fwd_adj()
+    torch.cuda.synchronize()
+    t_both = time.perf_counter() - t0
+    rep, rev, chunks = solver.last_adjoint_timing()
+    print(f"[bench] adjoint leg: forward+adjoint {t_both * 1e3:.0f} ms", file=sys.stderr, flush=True)
+    bad = int((~torch.isfinite(gy)).any(dim=0).sum())
+    return t_both / t_fwd, {
+        "workload": f"{args.mech} ROS3, mixed regimes, {n} cells, {k} sync steps of 5 min; "
+                    f"loss = sum over cells of final O3 + 0.1 NO2; gradients w.r.t. all initial "
+                    f"concentrations and all rate constants; sync-step states stored, stages recomputed",
+        "ncells": n, "sync_steps": k, "fwd_ms": t_fwd * 1e3, "fwd_adj_ms": t_both * 1e3,
+        "accepted_cell_steps": accepted,
+        "last_sync_step_replay_ms": rep, "last_sync_step_reverse_ms": rev, "tape_chunks": chunks,
+        "cells_without_gradient": bad}
 
 
 def main():
@@ -188,6 +251,13 @@ def main():
                      "kernel": "solve_kernel", "bytes_per_launch": alg_bytes},
         "fwd_adj_ratio": None,
     }
+    if not args.no_adj:
+        del y, y0, d, na, nj, sunlit
+        torch.cuda.empty_cache()
+        print("[bench] adjoint leg", file=sys.stderr, flush=True)
+        ratio, info = adjoint_leg(args, mech, solver, dev)
+        out["fwd_adj_ratio"] = ratio
+        out["adjoint"] = info
     if not args.no_cpu:
         print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args, mech, ncells, nlay_cells)

@@ -22,7 +22,7 @@ MECHS = {"cb6r5_ae7_aq": "cb6r5_ae7_aq.json", "cracmm2": "cracmm2.json"}
 # the first is the default unless tuning.json names another (written by
 # cmaq_amd/autotune.py from timings on the GPU)
 VARIANTS = {
-    "cb6r5_ae7_aq": [(110, 32), (100, 32), (56, 32)],
+    "cb6r5_ae7_aq": [(110, 32), (100, 32)],
     "cracmm2": [(110, 32), (100, 32)],
 }
 TUNING = os.path.join(HERE, "tuning.json")
@@ -90,7 +90,7 @@ def generate(names=None):
 
 
 def _compile(src, obj, host_only=False):
-    deps = [src, os.path.join(CSRC, "ros3_device.h"), os.path.join(CSRC, "ros3_registry.h"), os.path.join(CSRC, "ros3_solver_kernel.inc"),
+    deps = [src, os.path.join(CSRC, "ros3_device.h"), os.path.join(CSRC, "ros3_registry.h"), os.path.join(CSRC, "ros3_solver_kernel.inc"), os.path.join(CSRC, "ros3_adjoint_kernel.inc"),
             os.path.join(CSRC, "ros3_registry.h"), os.path.join(HERE, "..", "include", "cmaq_ros3.h")]
     if os.path.exists(obj) and all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps if os.path.exists(d)):
         return obj

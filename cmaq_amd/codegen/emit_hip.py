@@ -8,7 +8,9 @@ straight-line device functions built from programs.py's operation lists:
   rbsolver.F:377-381, rbdecomp.F);
 * ``solve1..3`` -- the three stage solves (rbsolve.F) with the stage
   right-hand sides of rbsolver.F:386-422 formed in place;
-* the persistent solver kernel (ros3_solver_kernel.inc) that calls them.
+* the persistent solver kernel (ros3_solver_kernel.inc) that calls them;
+* the discrete adjoint of one accepted step (programs.py: yp_ops ..
+  adj_b_ops) and the persistent reverse kernel (ros3_adjoint_kernel.inc).
 
 Registers are assigned by regalloc.allocate: at most REG_BUDGET doubles of a
 lane's workspace are register-resident, loads are issued LOOKAHEAD operations
@@ -23,7 +25,8 @@ night result is bit-identical to the reference's night pattern.
 
 import struct
 
-from .programs import feval_ops, jac_lu_ops, solve_ops
+from .programs import (adj_a_ops, adj_b_ops, adj_seed_ops, adj_u1_ops, adj_u2_ops, feval_ops,
+                       jac_lu_ops, solve_ops, tsolve_ops, yp_ops)
 from .regalloc import Op, allocate
 from .symbolic import analyse
 
@@ -117,7 +120,9 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     off = {}
     pos = 0
     for name, size in (("RK", nr), ("RATE", nr), ("Y", ns), ("YP", ns), ("F", ns),
-                       ("K1", ns), ("K2", ns), ("K3", ns), ("LU", nnz), ("VD", ns)):
+                       ("K1", ns), ("K2", ns), ("K3", ns), ("LU", nnz), ("VD", ns),
+                       ("LAM", ns), ("U1", ns), ("U2", ns), ("U3", ns), ("YB", ns),
+                       ("GK", nr), ("S2", nr), ("S3", nr), ("T", 8)):
         off[name] = pos
         pos += size
     nws = pos
@@ -159,6 +164,20 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
         body += _function(f"solve{stage}", params, items, off, budget)
         b("")
 
+    # discrete adjoint of one accepted step
+    for fname, ops, outs, params, dyn in (
+            ("yp_form", yp_ops(sym), {"YP"}, "", None),
+            ("adj_seed", adj_seed_ops(sym), {"LAM", "U3"}, "", None),
+            ("tsolve", tsolve_ops(sym, "X"), {"X"}, ", int x_off_", {"X": "x_off_"}),
+            ("adj_u2", adj_u2_ops(sym), {"U2"}, ", double x32", None),
+            ("adj_a", adj_a_ops(sym), {"YB", "S2", "S3"}, "", None),
+            ("adj_u1", adj_u1_ops(sym), {"U1"}, ", double x21, double x31", None),
+            ("adj_b", adj_b_ops(sym), {"LAM", "GK"}, "", None)):
+        items, cnt = allocate(ops, outs, budget, lookahead, entry_all)
+        stats[fname] = _stat(ops, cnt)
+        body += _function(fname, params, items, off, budget, dyn)
+        b("")
+
     out = []
     w = out.append
     w(f"// Generated by cmaq_amd/codegen/emit_hip.py for {m['mechname']}; do not edit.")
@@ -189,7 +208,7 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w("__device__ __noinline__ void load_cell(const SolveParams &P, int64_t cell, double *base_)")
     w("{")
     w("    ROS3_PROLOGUE")
-    w("    const int64_t n = P.ncells;")
+    w("    const int64_t n = P.stride;")
     w("    for (int r = 0; r < NR; ++r) ST(OFF_RK + r, P.rki[r * n + cell]);")
     for i in range(ns):
         w(f"    ST(OFF_Y + {i}, P.y[{sym.inew2old[i]} * n + cell]);")
@@ -197,13 +216,48 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w("__device__ __noinline__ void store_cell(const SolveParams &P, int64_t cell, double *base_)")
     w("{")
     w("    ROS3_PROLOGUE")
-    w("    const int64_t n = P.ncells;")
+    w("    const int64_t n = P.stride;")
     for i in range(ns):
         w(f"    P.y[{sym.inew2old[i]} * n + cell] = LD(OFF_Y + {i});")
+    w("}")
+    # adjoint cell load / store and step tapes (one record per accepted step:
+    # Y, K1, K2, K3 and the step size, contiguous per cell)
+    w(f"constexpr int TAPE_RS = {4 * ns + 2};")
+    w("__device__ __noinline__ void load_adj_cell(const SolveParams &P, int64_t cell, double *base_)")
+    w("{")
+    w("    ROS3_PROLOGUE")
+    w("    const int64_t n = P.stride;")
+    w("    for (int r = 0; r < NR; ++r) { ST(OFF_RK + r, P.rki[r * n + cell]); ST(OFF_GK + r, 0.0); }")
+    for i in range(ns):
+        w(f"    ST(OFF_LAM + {i}, P.lam[{sym.inew2old[i]} * n + cell]);")
+    w("}")
+    w("__device__ __noinline__ void store_adj_cell(const SolveParams &P, int64_t cell, double *base_)")
+    w("{")
+    w("    ROS3_PROLOGUE")
+    w("    const int64_t n = P.stride;")
+    for i in range(ns):
+        w(f"    P.lam[{sym.inew2old[i]} * n + cell] = LD(OFF_LAM + {i});")
+    w("    if (P.grad_rki)")
+    w("        for (int r = 0; r < NR; ++r) P.grad_rki[r * n + cell] += LD(OFF_GK + r);")
+    w("}")
+    w("__device__ __noinline__ void tape_store(double *base_, double *rec, double h)")
+    w("{")
+    w("    ROS3_PROLOGUE")
+    for k, arr in enumerate(("Y", "K1", "K2", "K3")):
+        w(f"    for (int i = 0; i < NS; ++i) rec[{k} * NS + i] = LD(OFF_{arr} + i);")
+    w("    rec[4 * NS] = h;")
+    w("}")
+    w("__device__ __noinline__ double tape_load(double *base_, const double *rec)")
+    w("{")
+    w("    ROS3_PROLOGUE")
+    for k, arr in enumerate(("Y", "K1", "K2", "K3")):
+        w(f"    for (int i = 0; i < NS; ++i) ST(OFF_{arr} + i, rec[{k} * NS + i]);")
+    w("    return rec[4 * NS];")
     w("}")
     w(f"const int32_t INEW2OLD_HOST[NS] = {{{', '.join(str(v) for v in sym.inew2old)}}};")
     w("")
     w("#include \"../ros3_solver_kernel.inc\"")
+    w("#include \"../ros3_adjoint_kernel.inc\"")
     w("#undef LD")
     w("#undef ST")
     w("#undef ROS3_PROLOGUE")
@@ -212,6 +266,8 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w(f"extern \"C\" const ros3::MechKernels ros3_kernels_{cname} = {{")
     w(f"    \"{m['mechname']}\", ros3::{cname}::STRUCT_HASH, ros3::{cname}::NS, ros3::{cname}::NR,")
     w(f"    ros3::{cname}::NNZ, {len(day.lu_ops)}, {sym.night.nnz}, {len(sym.night.lu_ops)}, ros3::{cname}::NWS,")
-    w(f"    (const void *)&ros3::{cname}::solve_kernel, ros3::{cname}::INEW2OLD_HOST,")
-    w(f"    \"{variant}\", {budget}, {lookahead}, ros3::{cname}::WAVES_PER_EU, {1 if is_default else 0}, {refill_min}}};")
+    w(f"    (const void *)&ros3::{cname}::solve_kernel_t<false>, ros3::{cname}::INEW2OLD_HOST,")
+    w(f"    \"{variant}\", {budget}, {lookahead}, ros3::{cname}::WAVES_PER_EU, {1 if is_default else 0}, {refill_min},")
+    w(f"    (const void *)&ros3::{cname}::solve_kernel_t<true>, (const void *)&ros3::{cname}::reverse_kernel,")
+    w(f"    ros3::{cname}::TAPE_RS}};")
     return "\n".join(out) + "\n", sym

@@ -181,3 +181,134 @@ def solve_ops(sym, stage):
             ops.append(Op("{0} = {0} - {1} * {2};", [(x, i), ("LU", s), (x, c)], True))
         ops.append(Op("{0} = {0} * {1};", [(x, i), ("VD", i)], True))
     return ops
+
+
+# ---------------------------------------------------------------------------
+# Discrete adjoint of one accepted ROS3 step (no reference counterpart).
+#
+# Forward step (rbsolver.F:357-473) with M = J(Y) - I/(GAM h):
+#   K1 = M^-1 (-F(Y)),  YP = Y + K1,  K2 = M^-1 (-F(YP) - x21 K1),
+#   K3 = M^-1 (-F(YP) - x31 K1 - x32 K2),  Ynew = max(Y + B1 K1 + B2 K2 + B3 K3, CONMIN)
+# Reverse with LAM = dL/dYnew (step sizes held fixed):
+#   LAM *= [Y + B.K > CONMIN];  U3 = M^-T (B3 LAM);  U2 = M^-T (B2 LAM - x32 U3)
+#   YB = J(YP)^T (-(U2 + U3));  U1 = M^-T (B1 LAM - x31 U3 - x21 U2 + YB)
+#   dL/dY = LAM + YB + J(Y)^T (-U1) - sum_s d/dY [U_s^T J(Y) K_s]
+#   dL/dk_r += -s1 p_r(Y) - (s2 + s3) p_r(YP) - sum_s s_s (grad p_r(Y) . K_s)
+# where s_s = (S^T U_s)_r (S the stoichiometric matrix) and p_r the product of
+# the reaction's reactant concentrations.  Reactions have at most two
+# reactants in the supported mechanisms.
+# ---------------------------------------------------------------------------
+
+def _check_bimolecular(sym):
+    for r in sym.day.active:
+        if len(sym.reactants[r]) > 2:
+            raise ValueError("adjoint code generation supports at most two reactants per reaction")
+
+
+def yp_ops(sym):
+    """YP = Y + K1, the forward's own expression (rbsolver.F:395-398)."""
+    return [Op("{0} = {1} + {2};", [("YP", i), ("Y", i), ("K1", i)], False) for i in range(sym.ns)]
+
+
+def adj_seed_ops(sym):
+    ops = []
+    for i in range(sym.ns):
+        ops.append(Op("{0} = ((({1} + B1 * {2}) + B2 * {3}) + B3 * {4}) > CONMIN ? {0} : 0.0;",
+                      [("LAM", i), ("Y", i), ("K1", i), ("K2", i), ("K3", i)], True))
+        ops.append(Op("{0} = B3 * {1};", [("U3", i), ("LAM", i)], False))
+    return ops
+
+
+def tsolve_ops(sym, x):
+    """x <- (L U)^T^-1 x with the factors of jac_lu: U^T by rows ascending,
+    then L^T by rows descending, both in scatter form."""
+    day = sym.day
+    ops = []
+    for i in range(sym.ns):
+        ops.append(Op("{0} = {0} * {1};", [(x, i), ("VD", i)], True))
+        for (s, c) in day.upper[i]:
+            ops.append(Op("{0} = {0} - {1} * {2};", [(x, c), ("LU", s), (x, i)], True))
+    for i in range(sym.ns - 1, -1, -1):
+        for (s, c) in day.lower[i]:
+            ops.append(Op("{0} = {0} - {1} * {2};", [(x, c), ("LU", s), (x, i)], True))
+    return ops
+
+
+def adj_u2_ops(sym):
+    return [Op("{0} = B2 * {1} - x32 * {2};", [("U2", i), ("LAM", i), ("U3", i)], False)
+            for i in range(sym.ns)]
+
+
+def _st_ops(sym, r, dst, src):
+    """dst = (S^T src)_r: reactants subtract, products add SC * src."""
+    ops = []
+    re = sym.reactants[r]
+    ops.append(Op("{0} = -{1};", [dst, (src, re[0])], False))
+    for a in re[1:]:
+        ops.append(Op("{0} = {0} - {1};", [dst, (src, a)], True))
+    for k, p in enumerate(sym.products[r]):
+        c = sym.sc[r][k]
+        if c == 1.0:
+            ops.append(Op("{0} = {0} + {1};", [dst, (src, p)], True))
+        else:
+            ops.append(Op("{0} = {0} + " + _lit(c) + " * {1};", [dst, (src, p)], True))
+    return ops
+
+
+def adj_a_ops(sym):
+    """S2, S3 per reaction and YB = J(YP)^T (-(U2 + U3))."""
+    _check_bimolecular(sym)
+    ops = [Op("{0} = 0.0;", [("YB", i)], False) for i in range(sym.ns)]
+    g = ("T", 0)
+    for r in sym.day.active:
+        re = sym.reactants[r]
+        ops += _st_ops(sym, r, ("S2", r), "U2")
+        ops += _st_ops(sym, r, ("S3", r), "U3")
+        ops.append(Op("{0} = -({1} * ({2} + {3}));", [g, ("RK", r), ("S2", r), ("S3", r)], False))
+        if len(re) == 1:
+            ops.append(Op("{0} = {0} + {1};", [("YB", re[0]), g], True))
+        else:
+            a, b = re
+            ops.append(Op("{0} = {0} + {1} * {2};", [("YB", a), g, ("YP", b)], True))
+            ops.append(Op("{0} = {0} + {1} * {2};", [("YB", b), g, ("YP", a)], True))
+    return ops
+
+
+def adj_u1_ops(sym):
+    return [Op("{0} = ((B1 * {1} - x31 * {2}) - x21 * {3}) + {4};",
+               [("U1", i), ("LAM", i), ("U3", i), ("U2", i), ("YB", i)], False) for i in range(sym.ns)]
+
+
+def adj_b_ops(sym):
+    """dL/dY into LAM and dL/dk accumulated into GK."""
+    _check_bimolecular(sym)
+    ops = [Op("{0} = {0} + {1};", [("YB", i), ("LAM", i)], True) for i in range(sym.ns)]
+    s1, s23, dk, t = ("T", 1), ("T", 2), ("T", 3), ("T", 4)
+    for r in sym.day.active:
+        re = sym.reactants[r]
+        S2, S3, rk = ("S2", r), ("S3", r), ("RK", r)
+        ops += _st_ops(sym, r, s1, "U1")
+        ops.append(Op("{0} = {1} + {2};", [s23, S2, S3], False))
+        if len(re) == 1:
+            a = re[0]
+            ops.append(Op("{0} = ((-({1} * {2}) - {3} * {4}) - {1} * {5}) - {6} * {7} - {8} * {9};",
+                          [dk, s1, ("Y", a), s23, ("YP", a), ("K1", a), S2, ("K2", a), S3, ("K3", a)], False))
+            ops.append(Op("{0} = {0} + {1};", [("GK", r), dk], True))
+            ops.append(Op("{0} = {0} - {1} * {2};", [("YB", a), rk, s1], True))
+            continue
+        a, b = re
+        ya, yb_ = ("Y", a), ("Y", b)
+        ops.append(Op("{0} = -({1} * ({2} * {3})) - {4} * ({5} * {6});",
+                      [dk, s1, ya, yb_, s23, ("YP", a), ("YP", b)], False))
+        for sig, K in ((s1, "K1"), (S2, "K2"), (S3, "K3")):
+            ops.append(Op("{0} = {0} - {1} * ({2} * {3} + {4} * {5});",
+                          [dk, sig, yb_, (K, a), ya, (K, b)], True))
+        ops.append(Op("{0} = {0} + {1};", [("GK", r), dk], True))
+        # slot a: other reactant b; slot b: other reactant a
+        for me, other in ((a, b), (b, a)):
+            ops.append(Op("{0} = {1} * {2} + {3} * {4} + {5} * {6};",
+                          [t, s1, ("K1", other), S2, ("K2", other), S3, ("K3", other)], False))
+            ops.append(Op("{0} = {0} - {1} * ({2} * {3} + {4});",
+                          [("YB", me), rk, s1, ("Y", other), t], True))
+    ops += [Op("{0} = {1};", [("LAM", i), ("YB", i)], False) for i in range(sym.ns)]
+    return ops

@@ -28,7 +28,7 @@ caller's, in the caller's order, so the schedule never changes a rounding.
 
 from collections import defaultdict
 
-RESERVE = 6     # registers kept free of prefetches: operands + result of one op
+RESERVE = 6     # registers kept free of prefetches (at least one op's operands + result)
 
 
 class Op:
@@ -55,6 +55,7 @@ def allocate(ops, outputs, budget, lookahead, defined_on_entry, lds_slots=0):
     defined_on_entry(slot): True if memory holds the slot's value on entry."""
     R, S = budget, lds_slots
     n = len(ops)
+    reserve = max([RESERVE] + [len(op.args) + 1 for op in ops])
     uses = defaultdict(list)
     for t, op in enumerate(ops):
         for s in _reads(op):
@@ -175,7 +176,7 @@ def allocate(ops, outputs, budget, lookahead, defined_on_entry, lds_slots=0):
                     written.add(ops[q].args[0])
             # operands produced inside the window are computed, not loaded;
             # prefetches never pin more than R - RESERVE registers
-            if sum(1 for r in range(R) if pinned[r] > t) >= R - RESERVE:
+            if sum(1 for r in range(R) if pinned[r] > t) + len(_reads(ops[pf])) > R - reserve:
                 break
             if not all(ensure(s, t, window, pf) for s in _reads(ops[pf]) if s not in written):
                 break

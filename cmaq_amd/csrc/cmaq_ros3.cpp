@@ -44,6 +44,7 @@ struct cmaq_ros3_ctx {
     int32_t refill_min = 0;
     int waves_per_cu = 0;
     int grid_blocks = 0;      // solver workgroups
+    int grid_replay = 0, grid_reverse = 0;   // adjoint kernels' workgroups
     int num_cus = 0;
     int64_t waves = 0;
     // device buffers
@@ -58,6 +59,13 @@ struct cmaq_ros3_ctx {
     void *d_small = nullptr;
     double *d_rki = nullptr;
     int64_t rki_cells = 0;
+    // adjoint tapes
+    double *d_tape = nullptr;
+    size_t tape_bytes = 0;
+    int64_t *d_tape_off = nullptr;
+    int64_t tape_off_cells = 0;
+    double replay_ms = 0.0, reverse_ms = 0.0;
+    int64_t adj_chunks = 0;
     // staging for host-pointer runs
     void *d_io = nullptr;
     size_t io_bytes = 0;
@@ -299,15 +307,25 @@ int dalloc(cmaq_ros3_ctx *c, T **p, size_t n)
 // Grid and workspace for one kernel variant: as many resident workgroups as
 // the variant's occupancy allows (or the caller's waves-per-CU cap), one
 // lane-private workspace per resident lane.
-static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
+static int grid_for(cmaq_ros3_ctx *c, const void *kernel, int *grid)
 {
     int per_cu = 0;
-    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mk->solve_kernel, kSolveBlock, 0));
+    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kSolveBlock, 0));
     if (per_cu < 1) per_cu = 1;
     const int wpb = kSolveBlock / kWave;
     if (c->waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, c->waves_per_cu / wpb));
-    const int grid = c->num_cus * per_cu;
-    const int64_t waves = (int64_t)grid * wpb;
+    *grid = c->num_cus * per_cu;
+    return 0;
+}
+
+static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
+{
+    int grid = 0, grid_a = 0, grid_b = 0;
+    if (int rc = grid_for(c, mk->solve_kernel, &grid)) return rc;
+    if (int rc = grid_for(c, mk->replay_kernel, &grid_a)) return rc;
+    if (int rc = grid_for(c, mk->reverse_kernel, &grid_b)) return rc;
+    const int wpb = kSolveBlock / kWave;
+    const int64_t waves = (int64_t)std::max(grid, std::max(grid_a, grid_b)) * wpb;
     const size_t need = (size_t)waves * mk->nws * kWave;
     const size_t have = c->mk && c->d_ws ? (size_t)c->waves * c->mk->nws * kWave : 0;
     if (need > have) {
@@ -318,6 +336,8 @@ static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
     }
     c->mk = mk;
     c->grid_blocks = grid;
+    c->grid_replay = grid_a;
+    c->grid_reverse = grid_b;
     c->waves = std::max<int64_t>(waves, have / ((size_t)mk->nws * kWave));
     return 0;
 }
@@ -424,6 +444,7 @@ extern "C" void cmaq_ros3_free(cmaq_ros3_ctx *c)
     hipSetDevice(c->device);
     hipFree(c->d_ops); hipFree(c->d_mul); hipFree(c->d_rtol); hipFree(c->d_atol);
     hipFree(c->d_ws); hipFree(c->d_small); hipFree(c->d_rki); hipFree(c->d_io);
+    hipFree(c->d_tape); hipFree(c->d_tape_off);
     for (hipEvent_t e : c->ev)
         if (e) hipEventDestroy(e);
     delete c;
@@ -517,14 +538,83 @@ extern "C" int cmaq_ros3_rates_device(cmaq_ros3_ctx *c, int64_t n, const double 
     return 0;
 }
 
+static void solve_params(cmaq_ros3_ctx *c, SolveParams &P, int64_t n, double chemstep, const double *rki,
+                         double *y, int32_t *nacc, int32_t *nrej)
+{
+    std::memset(&P, 0, sizeof P);
+    P.ncells = n; P.cell0 = 0; P.stride = n; P.chemstep = chemstep; P.rki = rki; P.y = y;
+    P.nacc = nacc; P.nrej = nrej;
+    P.rtol = c->d_rtol; P.atol = c->d_atol; P.ws = c->d_ws; P.queue = c->d_queue;
+    P.fail_flag = c->d_fail_flag; P.fail_cell = c->d_fail_cell; P.fail_info = c->d_fail_info;
+    P.fail_code = c->d_fail_code; P.max_attempts = c->max_attempts;
+    P.refill_min = c->refill_min > 0 ? c->refill_min : (c->mk->refill_min > 0 ? c->mk->refill_min : 2);
+}
+
+static const char *fail_text(int code)
+{
+    switch (code) {
+    case CMAQ_ROS3_E_CONVERGENCE: return "convergence failure";
+    case CMAQ_ROS3_E_STEPS: return "attempted-step limit";
+    case CMAQ_ROS3_E_TAPE: return "replay did not reproduce the forward's accepted steps";
+    default: return "failure";
+    }
+}
+
+// Reads the fail record after the stream has finished; returns its code.
+static int check_fail(cmaq_ros3_ctx *c, const unsigned char *small, cmaq_ros3_fail *fail)
+{
+    int32_t flag, code;
+    int64_t cell;
+    double info[3];
+    std::memcpy(&cell, small + 8, 8);
+    std::memcpy(info, small + 16, 24);
+    std::memcpy(&flag, small + 40, 4);
+    std::memcpy(&code, small + 44, 4);
+    if (!flag) return 0;
+    if (fail) { fail->code = code; fail->cell = cell; fail->maxerr = info[0]; fail->dt = info[1]; fail->tnow = info[2]; }
+    char buf[200];
+    std::snprintf(buf, sizeof buf, "%s in cell %lld (MAXERR %.4e, DT %.4e min, TNOW %.4e min)", fail_text(code),
+                  (long long)cell, info[0], info[1], info[2]);
+    return set_err(c, code, buf);
+}
+
+// Solver kernel over n cells with rate constants rki already on the device;
+// ev_from is the event recorded before the rates (or before the solve).
+static int launch_solve(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki, double *y,
+                        int32_t *nacc, int32_t *nrej, cmaq_ros3_fail *fail, hipStream_t s)
+{
+    HIP_TRY(c, hipEventRecord(c->ev[1], s));
+    SolveParams P;
+    solve_params(c, P, n, chemstep, rki, y, nacc, nrej);
+    void *args[] = {&P};
+    HIP_TRY(c, hipLaunchKernel(c->mk->solve_kernel, dim3(c->grid_blocks), dim3(kSolveBlock), args, 0, s));
+    HIP_TRY(c, hipEventRecord(c->ev[2], s));
+    unsigned char small[64];
+    HIP_TRY(c, hipMemcpyAsync(small, c->d_small, 64, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    float ms_r = 0.f, ms_s = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms_r, c->ev[0], c->ev[1]));
+    HIP_TRY(c, hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
+    c->rates_ms = ms_r;
+    c->solve_ms = ms_s;
+    return check_fail(c, small, fail);
+}
+
+static int check_run_args(cmaq_ros3_ctx *c, int64_t n, double chemstep, cmaq_ros3_fail *fail)
+{
+    if (fail) { fail->code = 0; fail->pad = 0; fail->cell = -1; fail->maxerr = fail->dt = fail->tnow = 0.0; }
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (n < 0 || !(chemstep > 0.0)) return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    return 0;
+}
+
 extern "C" int cmaq_ros3_run_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *temp,
                                     const double *pres, const double *h2o, const double *rj,
                                     const double *het, const double *sea, double *y, int32_t *nacc,
                                     int32_t *nrej, cmaq_ros3_fail *fail, void *stream)
 {
-    if (fail) { fail->code = 0; fail->pad = 0; fail->cell = -1; fail->maxerr = fail->dt = fail->tnow = 0.0; }
-    if (!c) return CMAQ_ROS3_E_ARG;
-    if (n < 0 || !(chemstep > 0.0) || (n > 0 && (!temp || !pres || !h2o || !y || (c->nphotab > 0 && !rj))))
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (n > 0 && (!temp || !pres || !h2o || !y || (c->nphotab > 0 && !rj)))
         return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
     if (n == 0) return 0;
     HIP_TRY(c, hipSetDevice(c->device));
@@ -541,39 +631,110 @@ extern "C" int cmaq_ros3_run_device(cmaq_ros3_ctx *c, int64_t n, double chemstep
     HIP_TRY(c, hipEventRecord(c->ev[0], s));
     int rc = launch_rates(c, n, temp, pres, h2o, rj, het, sea, c->d_rki, s);
     if (rc) return rc;
-    HIP_TRY(c, hipEventRecord(c->ev[1], s));
+    return launch_solve(c, n, chemstep, c->d_rki, y, nacc, nrej, fail, s);
+}
+
+extern "C" int cmaq_ros3_solve_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,
+                                      double *y, int32_t *nacc, int32_t *nrej, cmaq_ros3_fail *fail,
+                                      void *stream)
+{
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (n > 0 && (!rki || !y)) return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
+    HIP_TRY(c, hipEventRecord(c->ev[0], s));
+    return launch_solve(c, n, chemstep, rki, y, nacc, nrej, fail, s);
+}
+
+extern "C" int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,
+                                        const double *y0, const int32_t *nacc_fwd, double *lam,
+                                        double *grad_rki, cmaq_ros3_fail *fail, void *stream)
+{
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (n > 0 && (!rki || !y0 || !nacc_fwd || !lam)) return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    const int rs = c->mk->tape_rs;
+    // per-cell tape offsets from the forward's accepted-step counts; the cells
+    // are split into chunks whose tapes fit the tape budget
+    std::vector<int32_t> na((size_t)n);
+    HIP_TRY(c, hipMemcpyAsync(na.data(), nacc_fwd, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(c, hipMemGetInfo(&free_b, &total_b));
+    size_t budget = (size_t)((double)(free_b + c->tape_bytes) * 0.5);
+    if (const char *e = std::getenv("CMAQ_ROS3_TAPE_BYTES")) budget = std::min<size_t>(budget, std::strtoull(e, nullptr, 10));
+    const int64_t budget_rec = (int64_t)(budget / ((size_t)rs * sizeof(double)));
+    std::vector<int64_t> off((size_t)n);
+    std::vector<int64_t> chunk_beg{0}, chunk_rec;
+    int64_t acc = 0, max_rec = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t k = na[i] > 0 ? na[i] : 0;
+        if (k > budget_rec) return set_err(c, CMAQ_ROS3_E_NOMEM, "one cell's tape exceeds the tape budget");
+        if (acc + k > budget_rec) { chunk_rec.push_back(acc); chunk_beg.push_back(i); max_rec = std::max(max_rec, acc); acc = 0; }
+        off[i] = acc;
+        acc += k;
+    }
+    chunk_rec.push_back(acc);
+    max_rec = std::max(max_rec, acc);
+    const size_t need = (size_t)std::max<int64_t>(max_rec, 1) * rs * sizeof(double);
+    if (c->tape_bytes < need) {
+        hipFree(c->d_tape);
+        c->d_tape = nullptr;
+        c->tape_bytes = 0;
+        if (hipMalloc((void **)&c->d_tape, need) != hipSuccess) return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of the adjoint tape failed");
+        c->tape_bytes = need;
+    }
+    if (c->tape_off_cells < n) {
+        hipFree(c->d_tape_off);
+        c->d_tape_off = nullptr;
+        c->tape_off_cells = 0;
+        if (hipMalloc((void **)&c->d_tape_off, (size_t)n * sizeof(int64_t)) != hipSuccess)
+            return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of tape offsets failed");
+        c->tape_off_cells = n;
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->d_tape_off, off.data(), (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice, s));
     SolveParams P;
-    P.ncells = n; P.chemstep = chemstep; P.rki = c->d_rki; P.y = y; P.nacc = nacc; P.nrej = nrej;
-    P.rtol = c->d_rtol; P.atol = c->d_atol; P.ws = c->d_ws; P.queue = c->d_queue;
-    P.fail_flag = c->d_fail_flag; P.fail_cell = c->d_fail_cell; P.fail_info = c->d_fail_info;
-    P.fail_code = c->d_fail_code; P.max_attempts = c->max_attempts;
-    P.refill_min = c->refill_min > 0 ? c->refill_min : (c->mk->refill_min > 0 ? c->mk->refill_min : 2);
-    void *args[] = {&P};
-    HIP_TRY(c, hipLaunchKernel(c->mk->solve_kernel, dim3(c->grid_blocks), dim3(kSolveBlock), args, 0, s));
-    HIP_TRY(c, hipEventRecord(c->ev[2], s));
+    solve_params(c, P, n, chemstep, rki, const_cast<double *>(y0), nullptr, nullptr);
+    P.tape = c->d_tape; P.tape_off = c->d_tape_off; P.nacc_fwd = nacc_fwd; P.lam = lam; P.grad_rki = grad_rki;
+    HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
+    c->replay_ms = c->reverse_ms = 0.0;
+    c->adj_chunks = (int64_t)chunk_beg.size();
+    for (size_t k = 0; k < chunk_beg.size(); ++k) {
+        const int64_t b = chunk_beg[k], e = k + 1 < chunk_beg.size() ? chunk_beg[k + 1] : n;
+        P.cell0 = b;
+        P.ncells = e - b;
+        void *args[] = {&P};
+        HIP_TRY(c, hipMemsetAsync(c->d_queue, 0, 8, s));
+        HIP_TRY(c, hipEventRecord(c->ev[0], s));
+        HIP_TRY(c, hipLaunchKernel(c->mk->replay_kernel, dim3(c->grid_replay), dim3(kSolveBlock), args, 0, s));
+        HIP_TRY(c, hipEventRecord(c->ev[1], s));
+        HIP_TRY(c, hipMemsetAsync(c->d_queue, 0, 8, s));
+        HIP_TRY(c, hipLaunchKernel(c->mk->reverse_kernel, dim3(c->grid_reverse), dim3(kSolveBlock), args, 0, s));
+        HIP_TRY(c, hipEventRecord(c->ev[2], s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        float ms_a = 0.f, ms_b = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms_a, c->ev[0], c->ev[1]));
+        HIP_TRY(c, hipEventElapsedTime(&ms_b, c->ev[1], c->ev[2]));
+        c->replay_ms += ms_a;
+        c->reverse_ms += ms_b;
+    }
     unsigned char small[64];
     HIP_TRY(c, hipMemcpyAsync(small, c->d_small, 64, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
-    float ms_r = 0.f, ms_s = 0.f;
-    HIP_TRY(c, hipEventElapsedTime(&ms_r, c->ev[0], c->ev[1]));
-    HIP_TRY(c, hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
-    c->rates_ms = ms_r;
-    c->solve_ms = ms_s;
-    int32_t flag, code;
-    int64_t cell;
-    double info[3];
-    std::memcpy(&cell, small + 8, 8);
-    std::memcpy(info, small + 16, 24);
-    std::memcpy(&flag, small + 40, 4);
-    std::memcpy(&code, small + 44, 4);
-    if (flag) {
-        if (fail) { fail->code = code; fail->cell = cell; fail->maxerr = info[0]; fail->dt = info[1]; fail->tnow = info[2]; }
-        char buf[200];
-        std::snprintf(buf, sizeof buf, "%s in cell %lld (MAXERR %.4e, DT %.4e min, TNOW %.4e min)",
-                      code == CMAQ_ROS3_E_CONVERGENCE ? "convergence failure" : "attempted-step limit",
-                      (long long)cell, info[0], info[1], info[2]);
-        return set_err(c, code, buf);
-    }
+    return check_fail(c, small, fail);
+}
+
+extern "C" int cmaq_ros3_last_adjoint_timing(const cmaq_ros3_ctx *c, double *replay_ms, double *reverse_ms,
+                                             int64_t *chunks)
+{
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (replay_ms) *replay_ms = c->replay_ms;
+    if (reverse_ms) *reverse_ms = c->reverse_ms;
+    if (chunks) *chunks = c->adj_chunks;
     return 0;
 }
 

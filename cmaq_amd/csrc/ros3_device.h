@@ -230,11 +230,13 @@ __global__ __launch_bounds__(256) void rates_kernel(RatesParams P)
 
 // ---- persistent integration kernel ------------------------------------
 struct SolveParams {
-    int64_t ncells;
+    int64_t ncells;           // cells of this launch: cell0 .. cell0 + ncells - 1
+    int64_t cell0;
+    int64_t stride;           // leading dimension of the cell arrays
     double chemstep;
-    const double *rki;        // [nr][ncells]
-    double *y;                // [ns][ncells] mechanism order, in/out
-    int32_t *nacc, *nrej;     // [ncells] or nullptr
+    const double *rki;        // [nr][stride]
+    double *y;                // [ns][stride] mechanism order, in/out
+    int32_t *nacc, *nrej;     // [stride] or nullptr
     const double *rtol, *atol;// [ns] solver (new) order
     double *ws;               // workspace, waves * NWS * 64 doubles
     unsigned long long *queue;// next unassigned cell
@@ -244,6 +246,12 @@ struct SolveParams {
     int32_t *fail_code;
     int32_t max_attempts;
     int32_t refill_min;       // idle lanes needed before a wave refills
+    // adjoint: per-cell step tapes (records of Y, K1, K2, K3, h per accepted step)
+    double *tape;             // record storage of this launch
+    const int64_t *tape_off;  // [stride] first record of each cell
+    const int32_t *nacc_fwd;  // [stride] accepted steps of the forward run
+    double *lam;              // [ns][stride] mechanism order: dL/dy_end in, dL/dy0 out
+    double *grad_rki;         // [nr][stride] accumulated dL/drki, or nullptr
 };
 
 __device__ inline int lane_id() { return __lane_id(); }

@@ -16,6 +16,9 @@ struct MechKernels {
     int reg_budget, lookahead, waves_per_eu;
     int is_default;                   // chosen when no variant is requested
     int refill_min;                   // tuned refill threshold of this variant
+    const void *replay_kernel;        // solve_kernel writing per-step tapes
+    const void *reverse_kernel;       // discrete adjoint over the tapes
+    int tape_rs;                      // doubles per tape record
 };
 
 }  // namespace ros3

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "libcmaq_ros3.so")
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
 
-E_CODES = {-1: "ARG", -2: "MECH", -3: "HIP", -4: "CONVERGENCE", -5: "STEPS", -6: "NOMEM"}
+E_CODES = {-1: "ARG", -2: "MECH", -3: "HIP", -4: "CONVERGENCE", -5: "STEPS", -6: "NOMEM", -7: "TAPE"}
 
 
 class Ros3Error(RuntimeError):
@@ -81,6 +81,9 @@ def lib():
         L.cmaq_ros3_run.argtypes = [vp, C.c_int64, C.c_double] + [_f64p] * 7 + [_i32p, _i32p, C.POINTER(_Fail)]
         L.cmaq_ros3_run_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 9 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_rates_device.argtypes = [vp, C.c_int64] + [vp] * 7 + [vp]
+        L.cmaq_ros3_solve_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 4 + [C.POINTER(_Fail), vp]
+        L.cmaq_ros3_adjoint_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 5 + [C.POINTER(_Fail), vp]
+        L.cmaq_ros3_last_adjoint_timing.argtypes = [vp, _f64p, _f64p, C.POINTER(C.c_int64)]
         L.cmaq_ros3_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
         L.cmaq_ros3_species_order.argtypes = [vp, _i32p]
         L.cmaq_ros3_last_error.argtypes = [vp]
@@ -265,6 +268,56 @@ class Ros3Solver:
         if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
             raise self._err(rc, fail)
         return fail if rc != 0 else None
+
+    def _check(self, tensors, n):
+        import torch
+        for name, t, rows in tensors:
+            if t is None:
+                continue
+            if not (t.is_cuda and t.is_contiguous()):
+                raise ValueError(f"{name} must be a contiguous CUDA tensor")
+            want = torch.int32 if name in ("nacc", "nrej", "nacc_fwd") else torch.float64
+            if t.dtype != want:
+                raise ValueError(f"{name} must be {want}")
+            if t.numel() != rows * n:
+                raise ValueError(f"{name}: expected {rows} x {n} elements, got {t.numel()}")
+
+    def solve_device(self, chemstep, rki, y, nacc=None, nrej=None, stream=None, raise_on_fail=True):
+        """RBSOLVER with the rate constants given: rki [nr, n], y [ns, n]
+        (updated in place); see cmaq_ros3_solve_device."""
+        import torch
+        n = y.numel() // self.ns
+        self._check([("rki", rki, self.nr), ("y", y, self.ns), ("nacc", nacc, 1), ("nrej", nrej, 1)], n)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_ros3_solve_device(self._ctx, n, chemstep, _ptr(rki), _ptr(y), _ptr(nacc), _ptr(nrej),
+                                          C.byref(f), C.c_void_p(stream))
+        fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, fail)
+        return fail if rc != 0 else None
+
+    def adjoint_device(self, chemstep, rki, y0, nacc_fwd, lam, grad_rki=None, stream=None):
+        """Discrete adjoint of solve_device(chemstep, rki, y0): lam = dL/dy_end
+        is overwritten with dL/dy0; grad_rki (if given) accumulates dL/drki."""
+        import torch
+        n = y0.numel() // self.ns
+        self._check([("rki", rki, self.nr), ("y0", y0, self.ns), ("nacc_fwd", nacc_fwd, 1),
+                     ("lam", lam, self.ns), ("grad_rki", grad_rki, self.nr)], n)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_ros3_adjoint_device(self._ctx, n, chemstep, _ptr(rki), _ptr(y0), _ptr(nacc_fwd),
+                                            _ptr(lam), _ptr(grad_rki), C.byref(f), C.c_void_p(stream))
+        if rc != 0:
+            raise self._err(rc, {"code": rc, "cell": f.cell})
+
+    def last_adjoint_timing(self):
+        """(replay_ms, reverse_ms, chunks) of the last adjoint_device call."""
+        a, b, k = C.c_double(), C.c_double(), C.c_int64()
+        lib().cmaq_ros3_last_adjoint_timing(self._ctx, C.byref(a), C.byref(b), C.byref(k))
+        return a.value, b.value, k.value
 
     def rates_device(self, temp, pres_atm, h2o_ppm, rj, rki, het=None, seawater=None, stream=None):
         import torch

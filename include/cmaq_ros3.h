@@ -36,6 +36,7 @@ extern "C" {
 #define CMAQ_ROS3_E_CONVERGENCE  -4  /* DT fell below DTMIN (rbsolver.F:528-583)     */
 #define CMAQ_ROS3_E_STEPS        -5  /* attempted-step limit per cell exceeded       */
 #define CMAQ_ROS3_E_NOMEM        -6  /* device allocation failed                    */
+#define CMAQ_ROS3_E_TAPE         -7  /* adjoint replay differs from the forward run  */
 
 /*
  * Mechanism tables exactly as MODULE RXNS_DATA holds them
@@ -133,6 +134,41 @@ int cmaq_ros3_run_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min
                          const double *het_per_min, const double *seawater,
                          double *y_ppm, int32_t *n_accepted, int32_t *n_rejected,
                          cmaq_ros3_fail *fail, void *stream);
+
+/*
+ * RBSOLVER alone over ncells cells with the rate constants supplied
+ * (the RKI array of rbdata_mod.F, filled by CALC_RCONST at rbdriver.F:521):
+ * rki [nrxns][ncells] in ppm-min units, device memory.  Otherwise as
+ * cmaq_ros3_run_device.  Every cell uses the day pattern; for a cell with
+ * J = 0 the result is identical to the reference's night pattern.
+ */
+int cmaq_ros3_solve_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min,
+                           const double *rki, double *y_ppm, int32_t *n_accepted,
+                           int32_t *n_rejected, cmaq_ros3_fail *fail, void *stream);
+
+/*
+ * Discrete adjoint of one cmaq_ros3_solve_device call (no counterpart in the
+ * reference, whose sensitivity option is DDM-3D, ddm3d/DDM3D_CHEM.F:155-511).
+ * Given the call's inputs -- rki, the initial state y0 [ns][ncells] and the
+ * n_accepted it returned -- and lambda = dL/dy_end [ns][ncells] (mechanism
+ * order), overwrites lambda with dL/dy0 and adds dL/drki to grad_rki
+ * [nrxns][ncells] (NULL skips it).  The accepted step sizes are held fixed
+ * (the step-size controller is not differentiated).  The forward is replayed
+ * cell by cell into per-step tapes, in chunks of cells that fit half of the
+ * free device memory (CMAQ_ROS3_TAPE_BYTES lowers the budget); a replay that
+ * does not reproduce n_accepted returns CMAQ_ROS3_E_TAPE.  Cells with
+ * n_accepted < 0 (failed forward) get NaN in lambda.  Device memory, `stream`
+ * as in cmaq_ros3_run_device; returns after the stream has finished.
+ */
+int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min,
+                             const double *rki, const double *y0_ppm,
+                             const int32_t *n_accepted, double *lambda,
+                             double *grad_rki, cmaq_ros3_fail *fail, void *stream);
+
+/* Device time of the last cmaq_ros3_adjoint_device call: replay (tape
+ * writing) and reverse kernels, and the number of cell chunks. */
+int cmaq_ros3_last_adjoint_timing(const cmaq_ros3_ctx *ctx, double *replay_ms,
+                                  double *reverse_ms, int64_t *chunks);
 
 /*
  * CALC_RCONST alone (RXNS_FUNC_MODULE.F90:204-1116) on device arrays:

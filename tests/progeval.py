@@ -1,0 +1,93 @@
+"""Evaluate the generated operation lists (cmaq_amd/codegen/programs.py) in
+Python double precision.  Python floats round every operation separately, as
+the generated HIP (-ffp-contract=off) and the oracle (gcc -ffp-contract=off)
+do, so the evaluated programs can be compared with the oracle bit for bit."""
+import re
+
+from cmaq_amd.codegen import programs as P
+
+CONST = dict(GAM=0.43586652150845899941601945119356e+00, C21=-0.10156171083877702091975600115545e+01,
+             C31=0.40759956452537699824805835358067e+01, C32=0.92076794298330791242156818474003e+01,
+             B1=1.0, B2=0.61697947043828245592553615689730e+01, B3=-0.42772256543218573326238373806514e+00,
+             CONMIN=1.0e-30)
+_TERN = re.compile(r"^(.*?) = (.*) > CONMIN \? (.*) : 0\.0;$")
+
+
+def _name(a):
+    return f"{a[0]}[{a[1]}]" if isinstance(a, tuple) else a
+
+
+def compile_ops(ops, params=()):
+    lines = []
+    for op in ops:
+        s = op.fmt.format(*[_name(a) for a in op.args])
+        m = _TERN.match(s)
+        if m:
+            s = f"{m.group(1)} = {m.group(3)} if ({m.group(2)}) > CONMIN else 0.0"
+        lines.append("    " + s.rstrip(";"))
+    src = f"def _f(W, {', '.join(params) + ', ' if params else ''}**_):\n"
+    arrays = sorted({a[0] for op in ops for a in op.args if isinstance(a, tuple)})
+    src += "".join(f"    {a} = W['{a}']\n" for a in arrays)
+    src += "\n".join(lines) + "\n"
+    ns = dict(CONST)
+    exec(compile(src, "<ops>", "exec"), ns)
+    return ns["_f"]
+
+
+class Programs:
+    """The forward and adjoint programs of one mechanism as Python functions
+    on a workspace dict of lists (species in the solver's order)."""
+
+    def __init__(self, sym):
+        self.sym = sym
+        fe = P.feval_ops(sym, "scatter")
+        self.feval = compile_ops(fe)
+        self.jac_lu = compile_ops(P.jac_lu_ops(sym, "column"), ["gdtinv"])
+        self.solve = [compile_ops(P.solve_ops(sym, s), ["x1", "x2"]) for s in (1, 2, 3)]
+        self.yp = compile_ops(P.yp_ops(sym))
+        self.seed = compile_ops(P.adj_seed_ops(sym))
+        self.ts = {x: compile_ops(P.tsolve_ops(sym, x)) for x in ("U1", "U2", "U3")}
+        self.u2 = compile_ops(P.adj_u2_ops(sym), ["x32"])
+        self.a = compile_ops(P.adj_a_ops(sym))
+        self.u1 = compile_ops(P.adj_u1_ops(sym), ["x21", "x31"])
+        self.b = compile_ops(P.adj_b_ops(sym))
+
+    def workspace(self, rk, y):
+        ns, nr, nnz = self.sym.ns, self.sym.nr, self.sym.day.nnz
+        W = {k: [0.0] * ns for k in ("Y", "YP", "F", "K1", "K2", "K3", "VD", "LAM", "U1", "U2", "U3", "YB")}
+        W.update(RK=list(map(float, rk)), RATE=[0.0] * nr, LU=[0.0] * nnz, GK=[0.0] * nr,
+                 S2=[0.0] * nr, S3=[0.0] * nr, T=[0.0] * 8)
+        W["Y"] = list(map(float, y))
+        return W
+
+    def stages(self, W, h):
+        dtinv = 1.0 / h
+        gdtinv = dtinv * (1.0 / CONST["GAM"])
+        W["YS"] = W["Y"]
+        self.feval(W)
+        self.jac_lu(W, gdtinv)
+        self.solve[0](W, 0.0, 0.0)
+        self.yp(W)
+        W["YS"] = W["YP"]
+        self.feval(W)
+        self.solve[1](W, CONST["C21"] * dtinv, 0.0)
+        self.solve[2](W, CONST["C31"] * dtinv, CONST["C32"] * dtinv)
+
+    def ynew(self, W):
+        c = CONST
+        return [max(((y + c["B1"] * a) + c["B2"] * b) + c["B3"] * d, c["CONMIN"]) if
+                ((y + c["B1"] * a) + c["B2"] * b) + c["B3"] * d > c["CONMIN"] else c["CONMIN"]
+                for y, a, b, d in zip(W["Y"], W["K1"], W["K2"], W["K3"])]
+
+    def adjoint_step(self, W, h):
+        """W holds Y (step start), LAM (dL/dYnew); on exit LAM = dL/dY, GK += dL/dk."""
+        self.stages(W, h)
+        dtinv = 1.0 / h
+        self.seed(W)
+        self.ts["U3"](W)
+        self.u2(W, CONST["C32"] * dtinv)
+        self.ts["U2"](W)
+        self.a(W)
+        self.u1(W, CONST["C21"] * dtinv, CONST["C31"] * dtinv)
+        self.ts["U1"](W)
+        self.b(W)

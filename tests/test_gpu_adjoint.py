@@ -1,0 +1,156 @@
+"""GPU discrete adjoint against the oracle's adjoint and finite differences.
+
+The oracle adjoint is pinned on the CPU (tests/test_oracle_adjoint.py:
+central FD of the fixed-step map, closed form for linear decay).  Here the
+GPU's cmaq_ros3_adjoint_device is compared with it on the same rate
+constants, and the autograd wrapper with central FD (h = 1e-6 relative) of
+the oracle's fixed-step replay.  Bar: 1e-5 relative to the largest gradient
+entry of the cell.
+"""
+import numpy as np
+import pytest
+
+from cmaq_amd.inputs import make_cells
+
+pytestmark = pytest.mark.gpu
+
+RTOL, ATOL = 1.0e-3, 1.0e-7
+BAR = 1.0e-5
+
+
+def _dev(a, dtype=None):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda", dtype=dtype)
+
+
+@pytest.fixture(scope="module")
+def solver():
+    from cmaq_amd.ros3 import Ros3Solver
+    return Ros3Solver("cb6r5_ae7_aq", RTOL, ATOL)
+
+
+def _seed(mech, n):
+    names = mech["chemistry_spc"]
+    w = np.zeros((len(names), n))
+    w[names.index("O3")] = 1.0
+    w[names.index("NO2")] = 0.1
+    return w
+
+
+def _rates(solver, d):
+    import torch
+    n = d["temp"].size
+    rki = torch.zeros((solver.nr, n), dtype=torch.float64, device="cuda")
+    solver.rates_device(_dev(d["temp"]), _dev(d["pres"]), _dev(d["h2o"]), _dev(d["rj"]), rki)
+    return rki
+
+
+def test_solve_device_equals_run_device(cb6r5, solver):
+    import torch
+    n = 1000
+    d = make_cells(cb6r5, np.arange(n), seed=21, workload="mixed")
+    rki = _rates(solver, d)
+    y1, y2 = _dev(d["y"]).clone(), _dev(d["y"]).clone()
+    n1 = torch.zeros(n, dtype=torch.int32, device="cuda")
+    n2 = torch.zeros(n, dtype=torch.int32, device="cuda")
+    solver.run_device(5.0, _dev(d["temp"]), _dev(d["pres"]), _dev(d["h2o"]), _dev(d["rj"]), y1, n1,
+                      raise_on_fail=False)
+    solver.solve_device(5.0, rki, y2, n2, raise_on_fail=False)
+    assert torch.equal(y1, y2) and torch.equal(n1, n2)
+
+
+@pytest.mark.parametrize("workload,seed", [("mixed", 22), ("clean", 23)])
+def test_adjoint_matches_oracle(cb6r5, oracle_cb6r5, solver, workload, seed):
+    import torch
+    n = 512
+    d = make_cells(cb6r5, np.arange(n), seed=seed, workload=workload)
+    rki = _rates(solver, d)
+    y0 = _dev(d["y"])
+    y = y0.clone()
+    na = torch.zeros(n, dtype=torch.int32, device="cuda")
+    solver.solve_device(5.0, rki, y, na, raise_on_fail=False)
+    w = _seed(cb6r5, n)
+    lam = _dev(w).clone()
+    gk = torch.zeros_like(rki)
+    solver.adjoint_device(5.0, rki, y0, na, lam, gk)
+    ref = oracle_cb6r5.adjoint_run(5.0, rki.cpu().numpy(), d["y"], w, RTOL, ATOL)
+    nag, lg, gg = na.cpu().numpy(), lam.cpu().numpy(), gk.cpu().numpy()
+    ok = (nag >= 0) & (ref["nacc"] == nag)
+    assert ok.mean() >= 0.99
+    assert np.isnan(lg[:, nag < 0]).all()
+    for c in np.flatnonzero(ok):
+        sl, sg = lg[:, c] * d["y"][:, c], gg[:, c] * rki[:, c].cpu().numpy()
+        rl, rg = ref["lam"][:, c] * d["y"][:, c], ref["gk"][:, c] * rki[:, c].cpu().numpy()
+        assert np.abs(sl - rl).max() <= 1e-8 * np.abs(rl).max(), c
+        assert np.abs(sg - rg).max() <= 1e-8 * np.abs(rg).max(), c
+    print(f"adjoint {workload}: cells {n}, step counts equal {ok.mean():.4f}, "
+          f"replay/reverse ms {solver.last_adjoint_timing()}")
+
+
+def test_autograd_two_sync_steps_vs_fd(cb6r5, oracle_cb6r5, solver):
+    """torch.autograd through 2 sync steps; loss = sum(O3 + 0.1 NO2).
+    dL/dln y0 and dL/dln k checked against central FD of the oracle's
+    fixed-step replay on 16 cells, top entries per cell."""
+    import torch
+    from cmaq_amd.adjoint import integrate
+    n = 16
+    d = make_cells(cb6r5, np.arange(n), seed=24, workload="mixed")
+    rki = _rates(solver, d)
+    y0 = _dev(d["y"]).requires_grad_(True)
+    logk = torch.log(rki).requires_grad_(True)   # exp(log k) is not k bit for bit: use it on both sides
+    k_used = logk.exp()
+    yend = integrate(solver, y0, k_used, 5.0, 2)
+    names = cb6r5["chemistry_spc"]
+    loss = (yend[names.index("O3")] + 0.1 * yend[names.index("NO2")]).sum()
+    loss.backward()
+    gy = y0.grad.cpu().numpy() * d["y"]
+    gl = logk.grad.cpu().numpy()
+    kk = k_used.detach().cpu().numpy()
+    w = _seed(cb6r5, 1)[:, 0]
+    o = oracle_cb6r5
+    for c in range(n):
+        k = kk[:, c]
+        h1, y1 = o.steps_cell(5.0, k, d["y"][:, c], RTOL, ATOL)
+        h2, _ = o.steps_cell(5.0, k, y1, RTOL, ATOL)
+
+        def L(y, kv):
+            return w @ o.replay_cell(kv, o.replay_cell(kv, y, h1), h2)
+
+        for j in np.argsort(-np.abs(gy[:, c]))[:4]:
+            e = 1e-6 * d["y"][j, c]
+            yp, ym = d["y"][:, c].copy(), d["y"][:, c].copy()
+            yp[j] += e
+            ym[j] -= e
+            fd = (L(yp, k) - L(ym, k)) / (2 * e) * d["y"][j, c]
+            assert abs(gy[j, c] - fd) <= BAR * np.abs(gy[:, c]).max(), (c, names[j], gy[j, c], fd)
+        for r in np.argsort(-np.abs(gl[:, c]))[:4]:
+            e = 1e-6 * k[r]
+            kp, km = k.copy(), k.copy()
+            kp[r] += e
+            km[r] -= e
+            fd = (L(d["y"][:, c], kp) - L(d["y"][:, c], km)) / (2 * e) * k[r]
+            assert abs(gl[r, c] - fd) <= BAR * np.abs(gl[:, c]).max(), (c, r, gl[r, c], fd)
+
+
+def test_adjoint_in_chunks_equals_single_pass(cb6r5, solver, monkeypatch):
+    """A tape budget too small for all cells splits them into chunks; the
+    gradients are the same bit for bit."""
+    import torch
+    n = 700
+    d = make_cells(cb6r5, np.arange(n), seed=25, workload="mixed")
+    rki = _rates(solver, d)
+    y0 = _dev(d["y"])
+    y = y0.clone()
+    na = torch.zeros(n, dtype=torch.int32, device="cuda")
+    solver.solve_device(5.0, rki, y, na, raise_on_fail=False)
+    outs = []
+    for budget in (None, 64 * 1024 * 1024):
+        if budget:
+            monkeypatch.setenv("CMAQ_ROS3_TAPE_BYTES", str(budget))
+        lam = _dev(_seed(cb6r5, n)).clone()
+        gk = torch.zeros_like(rki)
+        solver.adjoint_device(5.0, rki, y0, na, lam, gk)
+        outs.append((lam.cpu().numpy(), gk.cpu().numpy(), solver.last_adjoint_timing()[2]))
+    assert outs[0][2] == 1 and outs[1][2] > 1
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])

@@ -1,0 +1,62 @@
+"""The generated operation lists, evaluated in Python double precision,
+against the oracle: the forward programs bit for bit (same per-entry rounding
+order), the adjoint programs to rounding (different summation order)."""
+import numpy as np
+import pytest
+
+from cmaq_amd.codegen.symbolic import analyse
+from cmaq_amd.inputs import make_cells
+from progeval import Programs
+
+
+@pytest.fixture(scope="module")
+def prog_cb6r5(cb6r5):
+    return Programs(analyse(cb6r5))
+
+
+def _cell(mech, o, seed):
+    d = make_cells(mech, np.arange(1), seed=seed, workload="mixed")
+    rki, _ = o.rates(d["temp"], d["pres"], d["h2o"], d["rj"])
+    return rki[:, 0].copy(), d["y"][:, 0].copy()
+
+
+def test_forward_programs_bit_exact(cb6r5, oracle_cb6r5, prog_cb6r5):
+    o, pg = oracle_cb6r5, prog_cb6r5
+    inew = o.inew2old() - 1
+    for seed in (1, 4):
+        rki, y0 = _cell(cb6r5, o, seed)
+        h, yend = o.steps_cell(5.0, rki, y0)
+        W = pg.workspace(rki, y0[inew])
+        for hn in h:
+            pg.stages(W, hn)
+            W["Y"] = pg.ynew(W)
+        assert np.array_equal(np.array(W["Y"]), yend[inew])
+
+
+def test_adjoint_programs_match_oracle(cb6r5, oracle_cb6r5, prog_cb6r5):
+    o, pg = oracle_cb6r5, prog_cb6r5
+    inew = o.inew2old() - 1
+    names = cb6r5["chemistry_spc"]
+    w = np.zeros(len(names))
+    w[names.index("O3")] = 1.0
+    w[names.index("NO2")] = 0.1
+    for seed in (2, 5):
+        rki, y0 = _cell(cb6r5, o, seed)
+        h, _ = o.steps_cell(5.0, rki, y0)
+        lam_ref, gk_ref = o.adjoint_cell(rki, y0, h, w)
+        # states along the trajectory
+        W = pg.workspace(rki, y0[inew])
+        states = []
+        for hn in h:
+            states.append(list(W["Y"]))
+            pg.stages(W, hn)
+            W["Y"] = pg.ynew(W)
+        W["LAM"] = list(w[inew])
+        for n in range(len(h) - 1, -1, -1):
+            W["Y"] = list(states[n])
+            pg.adjoint_step(W, h[n])
+        lam = np.empty(len(names))
+        lam[inew] = W["LAM"]
+        gk = np.array(W["GK"])
+        np.testing.assert_allclose(lam, lam_ref, rtol=1e-10, atol=1e-12 * np.abs(lam_ref).max())
+        np.testing.assert_allclose(gk, gk_ref, rtol=1e-10, atol=1e-12 * np.abs(gk_ref).max())
