@@ -16,6 +16,10 @@ The JSON line also carries
   cpu_baseline - the CPU oracle (a C restatement of the reference ROS3, built
                  with the reference's gcc flags) timed on a bounded sample of
                  the same cells on the host cores, in stock 50-cell blocks;
+  sync_loop    - the uneven-stiffness workload: 1,048,576 mixed-regime cells
+                 (40 % clean day, 30 % night, 30 % near-source), 12 sync steps
+                 of 5 min replayed from one captured hipGraph (rates -> solve
+                 -> checkpoint per step);
   fwd_adj_ratio - forward + discrete adjoint over forward alone, on the
                  differentiation workload: 262,144 mixed-regime cells, 12
                  sync steps of 5 min, loss = sum(O3 + 0.1 NO2) at the end,
@@ -50,6 +54,9 @@ def _args():
     p.add_argument("--adj-cells", type=int, default=262144)
     p.add_argument("--adj-sync-steps", type=int, default=12)
     p.add_argument("--no-adj", action="store_true")
+    p.add_argument("--loop-cells", type=int, default=1048576)
+    p.add_argument("--loop-sync-steps", type=int, default=12)
+    p.add_argument("--no-loop", action="store_true")
     return p.parse_args()
 
 
@@ -96,6 +103,37 @@ def cpu_baseline(args, mech, ncells, nlay_cells):
                       f"one 5-min step, 50-cell blocks (stock BLKSIZE, no stiffness pre-sort), "
                       f"{workers} processes on disjoint cells; slowest process {busy:.1f} s, wall {wall:.1f} s",
             "failed_blocks": int(sum(1 for r in res if r[3] != 0))}
+
+
+def loop_leg(args, mech, solver, dev):
+    """12 sync steps of the mixed workload through the captured graph."""
+    import torch
+    from cmaq_amd.inputs import make_cells_torch
+    n, k = args.loop_cells, args.loop_sync_steps
+    d = make_cells_torch(mech, n, seed=args.seed + 2, workload="mixed", device=dev)
+    y = d["y"].clone()
+    na = torch.empty((k, n), dtype=torch.int32, device=dev)
+    nj = torch.empty((k, n), dtype=torch.int32, device=dev)
+    args_ = (5.0, k, d["temp"], d["pres"], d["h2o"], d["rj"], y, None, na, nj)
+    solver.run_steps_device(*args_, raise_on_fail=False)       # captures the graph
+    y.copy_(d["y"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.run_steps_device(*args_, raise_on_fail=False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dev_ms = solver.last_steps_timing()
+    ok = (na >= 0).all(dim=0)
+    att = int((na.clamp(min=0) + nj).sum())
+    per_cell = (na.clamp(min=0) + nj).sum(dim=0)[ok].to(torch.float64)
+    q = torch.quantile(per_cell, torch.tensor([0.01, 0.5, 0.99], dtype=torch.float64, device=dev)).tolist()
+    return {"workload": f"{args.mech} ROS3, mixed regimes (40% clean day, 30% night, 30% near-source with "
+                        f"NO/NO2 log-uniform on [10,500] ppb and VOC x10), {n} cells, {k} sync steps of "
+                        f"5 min, one hipGraph per sync step (rates -> solve -> checkpoint)",
+            "ncells": n, "sync_steps": k, "ms": wall * 1e3, "device_ms": dev_ms,
+            "attempted_cell_steps_per_s": att / wall,
+            "attempted_steps_per_cell_p01_p50_p99": q,
+            "cells_failing_some_step": int((~ok).sum())}
 
 
 def adjoint_leg(args, mech, solver, dev):
@@ -251,9 +289,14 @@ def main():
                      "kernel": "solve_kernel", "bytes_per_launch": alg_bytes},
         "fwd_adj_ratio": None,
     }
-    if not args.no_adj:
+    if not (args.no_adj and args.no_loop):
         del y, y0, d, na, nj, sunlit
         torch.cuda.empty_cache()
+    if not args.no_loop:
+        print("[bench] sync-loop leg", file=sys.stderr, flush=True)
+        out["sync_loop"] = loop_leg(args, mech, solver, dev)
+        torch.cuda.empty_cache()
+    if not args.no_adj:
         print("[bench] adjoint leg", file=sys.stderr, flush=True)
         ratio, info = adjoint_leg(args, mech, solver, dev)
         out["fwd_adj_ratio"] = ratio

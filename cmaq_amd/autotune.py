@@ -15,14 +15,14 @@ import os
 import sys
 
 
-def tune(mech_name, ncells, workload="regional", seed=1, refills=(1, 2, 4), log=print):
+def tune(mech_name, ncells, workload="regional", seed=1, refills=(1, 2, 4), log=print, svoc_ppb=None):
     import torch
     from .inputs import make_cells_torch
     from .ros3 import Ros3Solver, load_mechanism
     mech = load_mechanism(mech_name)
     dev = torch.device("cuda:0")
     d = make_cells_torch(mech, ncells, seed=seed, workload=workload,
-                         nlay_cells=max(1, ncells // 35), device=dev)
+                         nlay_cells=max(1, ncells // 35), svoc_ppb=svoc_ppb, device=dev)
     y = torch.empty_like(d["y"])
     na = torch.zeros(ncells, dtype=torch.int32, device=dev)
     nj = torch.zeros(ncells, dtype=torch.int32, device=dev)
@@ -53,13 +53,16 @@ def main():
     p.add_argument("--mech", action="append", default=None)
     p.add_argument("--ncells", type=int, default=1048576)
     p.add_argument("--out", default="tuning.json")
+    p.add_argument("--workload", default="regional", choices=["regional", "clean", "mixed"])
+    p.add_argument("--svoc", action="store_true", help="VOC/SVOC species log-uniform on [1e-5, 5] ppb")
     a = p.parse_args()
     table = {}
     if os.path.exists(a.out):
         with open(a.out) as f:
             table = json.load(f)
     for m in a.mech or ["cb6r5_ae7_aq"]:
-        table[m] = tune(m, a.ncells, log=lambda s: print(s, file=sys.stderr, flush=True))
+        table[m] = tune(m, a.ncells, workload=a.workload, svoc_ppb=(1e-5, 5.0) if a.svoc else None,
+                        log=lambda s: print(s, file=sys.stderr, flush=True))
         with open(a.out, "w") as f:
             json.dump(table, f, indent=1)
     print(json.dumps({m: {k: v for k, v in t.items() if k != "results"} for m, t in table.items()}))

@@ -18,13 +18,20 @@ GEN = os.path.join(CSRC, "gen")
 OBJ = os.path.join(GEN, "obj")
 LIB = os.path.join(HERE, "libcmaq_ros3.so")
 MECHS = {"cb6r5_ae7_aq": "cb6r5_ae7_aq.json", "cracmm2": "cracmm2.json"}
-# code-generation variants per mechanism: (register budget, load lookahead);
-# the first is the default unless tuning.json names another (written by
-# cmaq_amd/autotune.py from timings on the GPU)
+# code-generation variants per mechanism: (register budget, load lookahead,
+# LU schedule of programs.jac_lu_ops); the first is the default unless
+# tuning.json names another (written by cmaq_amd/autotune.py from timings on
+# the GPU)
 VARIANTS = {
-    "cb6r5_ae7_aq": [(110, 32), (100, 32)],
-    "cracmm2": [(110, 32), (100, 32)],
+    "cb6r5_ae7_aq": [(110, 32, "column"), (110, 32, "tile10x10"), (110, 8, "tile10x10")],
+    "cracmm2": [(110, 32, "column"), (110, 32, "tile12x12"), (110, 8, "tile12x12")],
 }
+
+
+def variant_name(b, l, order):
+    suffix = "" if order == "column" else "_t" + order[4:]
+    return f"b{b}_l{l}{suffix}"
+
 TUNING = os.path.join(HERE, "tuning.json")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -55,9 +62,8 @@ def tuned(name):
     if os.path.exists(TUNING):
         with open(TUNING) as f:
             table = json.load(f)
-    b, l = VARIANTS[name][0]
     entry = table.get(name, {})
-    return entry.get("variant", f"b{b}_l{l}"), int(entry.get("refill_min", 2))
+    return entry.get("variant", variant_name(*VARIANTS[name][0])), int(entry.get("refill_min", 2))
 
 
 def generate(names=None):
@@ -67,11 +73,11 @@ def generate(names=None):
     for name in names:
         mech = load_mech(name)
         dflt, refill = tuned(name)
-        for (b, l) in VARIANTS[name]:
-            var = f"b{b}_l{l}"
+        for (b, l, order) in VARIANTS[name]:
+            var = variant_name(b, l, order)
             cname = f"{name}_{var}"
             text, _ = emit(mech, cname, budget=b, lookahead=l, variant=var, is_default=(var == dflt),
-                           refill_min=refill)
+                           refill_min=refill, jac_order=order)
             path = os.path.join(GEN, f"ros3_{cname}.hip")
             _write_if_changed(path, text)
             srcs.append(path)

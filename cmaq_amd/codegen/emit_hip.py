@@ -107,9 +107,11 @@ def waves_per_eu(budget):
     return max(1, min(8, 512 // vgprs))
 
 
-def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, refill_min=0):
+def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, refill_min=0,
+         jac_order=None):
     budget = budget or REG_BUDGET
     lookahead = lookahead or LOOKAHEAD
+    jac_order = jac_order or JAC_ORDER
     sym = analyse(m)
     if sym.ischan != sym.ns:
         raise ValueError("species without Jacobian entries are not supported")
@@ -139,7 +141,7 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     body += _function("feval", ", int ys_off_", items, off, budget, {"YS": "ys_off_"})
     b("")
 
-    ops = jac_lu_ops(sym, JAC_ORDER)
+    ops = jac_lu_ops(sym, jac_order)
     items, cnt = allocate(ops, {"LU", "VD"}, budget, lookahead, entry_all)
     stats["jac_lu"] = _stat(ops, cnt)
     body += _function("jac_lu", ", double gdtinv", items, off, budget)
@@ -181,7 +183,7 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     out = []
     w = out.append
     w(f"// Generated by cmaq_amd/codegen/emit_hip.py for {m['mechname']}; do not edit.")
-    w(f"// register budget {budget} doubles, load lookahead {lookahead} operations;")
+    w(f"// register budget {budget} doubles, load lookahead {lookahead} operations, LU schedule {jac_order};")
     w("// per call (operations, loads, stores): " + ", ".join(f"{k} {v}" for k, v in stats.items()))
     w("#include \"../ros3_device.h\"")
     w(f"namespace ros3 {{ namespace {cname} {{")

@@ -141,6 +141,94 @@ def jac_lu_ops(sym, order="column"):
             for s in day.l_scale[j]:
                 ops.append(Op("{0} = {0} * {1};", [("LU", s), ("VD", j)], True))
         return ops
+    if order.startswith("tile"):
+        # 2-D tiles of br x bc entries, tiles row-block by row-block: a tile's
+        # entries take their updates k by k (each L(i,k) and U(k,j) loaded once
+        # per tile), a column's pivot and L scaling come when k reaches it.
+        # Every operand is final when used: L(i,k) lies in an earlier tile of
+        # the same row block (or this one), U(k,j) in an earlier row block of
+        # the same column block (or this tile).
+        br, bc = (int(v) for v in order[4:].split("x"))
+        upd = {}
+        for (ij, ik, kj) in day.lu_ops:
+            upd.setdefault(ij, []).append((day.cols[ik], ik, kj))
+        tiles = {}
+        for s in range(day.nnz):
+            tiles.setdefault((day.rows[s] // br, day.cols[s] // bc), []).append(s)
+        finished = set()
+
+        def finish(k, in_tile):
+            out = []
+            if (k, k) in day.slot and day.slot[(k, k)] in in_tile and k not in finished:
+                out.append(Op("{0} = 1.0 / {1};", [("VD", k), ("LU", day.diag[k])], False))
+                finished.add(k)
+            for s in day.l_scale[k]:
+                if s in in_tile:
+                    out.append(Op("{0} = {0} * {1};", [("LU", s), ("VD", k)], True))
+            return out
+
+        for key in sorted(tiles):
+            ents = sorted(tiles[key], key=lambda s: (day.rows[s], day.cols[s]))
+            in_tile = set(ents)
+            for s in ents:
+                ops += _entry_ops(sym, terms, s, day.diag[day.cols[s]])
+            cols = sorted({day.cols[s] for s in ents})
+            by_k = {}
+            for s in ents:
+                for (k, ik, kj) in upd.get(s, []):
+                    by_k.setdefault(k, []).append((s, ik, kj))
+            done_cols = set()
+            for k in sorted(by_k):
+                for c in cols:
+                    if c <= k and c not in done_cols:
+                        ops += finish(c, in_tile)
+                        done_cols.add(c)
+                for (s, ik, kj) in by_k[k]:
+                    ops.append(Op("{0} = {0} - {1} * {2};", [("LU", s), ("LU", ik), ("LU", kj)], True))
+            for c in cols:
+                if c not in done_cols:
+                    ops += finish(c, in_tile)
+        return ops
+    if order.startswith("group"):
+        # columns in groups of g: the updates of a group are applied k by k, so
+        # each L(i,k) is loaded once per group rather than once per column;
+        # column k of the group is finished (pivot, L scaling) when k is reached
+        g = int(order[5:] or 2)
+        col_slots = [[] for _ in range(n)]
+        for s in range(day.nnz):
+            col_slots[day.cols[s]].append(s)
+
+        def finish(j):
+            out = [Op("{0} = 1.0 / {1};", [("VD", j), ("LU", day.diag[j])], False)]
+            for s in day.l_scale[j]:
+                out.append(Op("{0} = {0} * {1};", [("LU", s), ("VD", j)], True))
+            return out
+
+        for j0 in range(0, n, g):
+            grp = list(range(j0, min(n, j0 + g)))
+            for j in grp:
+                for s in col_slots[j]:
+                    ops += _entry_ops(sym, terms, s, day.diag[j])
+            by_k = {}
+            for j in grp:
+                beg, end = day.lu_col_ops[j]
+                for (ij, ik, kj) in day.lu_ops[beg:end]:
+                    by_k.setdefault(day.cols[ik], []).append((j, ij, ik, kj))
+            done = set()
+            for k in sorted(by_k):
+                if k in grp and k not in done:
+                    for j in grp:
+                        if j < k and j not in done:
+                            ops += finish(j)
+                            done.add(j)
+                    ops += finish(k)
+                    done.add(k)
+                for (j, ij, ik, kj) in by_k[k]:
+                    ops.append(Op("{0} = {0} - {1} * {2};", [("LU", ij), ("LU", ik), ("LU", kj)], True))
+            for j in grp:
+                if j not in done:
+                    ops += finish(j)
+        return ops
     # row order: row i's entries, then for k ascending: finish L(i,k), apply its updates
     by_row = [dict() for _ in range(n)]       # i -> k -> [(ij, kj)]
     for (ij, ik, kj) in day.lu_ops:

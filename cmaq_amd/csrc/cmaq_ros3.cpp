@@ -72,6 +72,14 @@ struct cmaq_ros3_ctx {
     // kernel timing of the last run: rates start/end, solve end
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     double rates_ms = 0.0, solve_ms = 0.0;
+    // captured sync-step sequence (cmaq_ros3_run_steps_device)
+    hipStream_t gstream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<const void *> graph_key;
+    int32_t *d_step_na = nullptr, *d_step_nr = nullptr;
+    int64_t step_cells = 0;
+    double steps_ms = 0.0;
     std::string err;
 };
 
@@ -445,6 +453,10 @@ extern "C" void cmaq_ros3_free(cmaq_ros3_ctx *c)
     hipFree(c->d_ops); hipFree(c->d_mul); hipFree(c->d_rtol); hipFree(c->d_atol);
     hipFree(c->d_ws); hipFree(c->d_small); hipFree(c->d_rki); hipFree(c->d_io);
     hipFree(c->d_tape); hipFree(c->d_tape_off);
+    hipFree(c->d_step_na); hipFree(c->d_step_nr);
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    if (c->gstream) hipStreamDestroy(c->gstream);
     for (hipEvent_t e : c->ev)
         if (e) hipEventDestroy(e);
     delete c;
@@ -782,4 +794,137 @@ extern "C" int cmaq_ros3_run(cmaq_ros3_ctx *c, int64_t n, double chemstep, const
     if (nacc) HIP_TRY(c, hipMemcpy(nacc, d_na, (size_t)n * 4, hipMemcpyDeviceToHost));
     if (nrej) HIP_TRY(c, hipMemcpy(nrej, d_nr, (size_t)n * 4, hipMemcpyDeviceToHost));
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Sync-step loop captured in a hipGraph: per sync step the graph resets the
+// cell queue, evaluates the rate constants, integrates every cell and writes
+// the checkpoint (state and step counts of that sync step).  Only kernels are
+// captured (no memset nodes); the step index lives in device memory and
+// advances inside the graph, so one instantiated graph serves every step.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct CkptParams {
+    int64_t n, ns;
+    const double *y;
+    double *y_ckpt;                 // [nsteps][ns][n] or nullptr
+    const int32_t *na, *nr;
+    int32_t *na_all, *nr_all;       // [nsteps][n] or nullptr
+    const int32_t *step;            // current sync step (device)
+};
+
+__global__ void queue_reset_kernel(unsigned long long *queue) { *queue = 0ull; }
+
+// queue, fail record and step counter: the first 64 bytes of the small buffer
+__global__ void small_reset_kernel(unsigned long long *small)
+{
+    if (threadIdx.x < 8) small[threadIdx.x] = 0ull;
+}
+
+__global__ void checkpoint_kernel(CkptParams P)
+{
+    const int64_t s = *P.step;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (P.y_ckpt) {
+        double *dst = P.y_ckpt + s * P.ns * P.n;
+        for (int64_t i = t0; i < P.ns * P.n; i += stride) dst[i] = P.y[i];
+    }
+    for (int64_t i = t0; i < P.n; i += stride) {
+        if (P.na_all) P.na_all[s * P.n + i] = P.na[i];
+        if (P.nr_all) P.nr_all[s * P.n + i] = P.nr[i];
+    }
+}
+
+__global__ void step_advance_kernel(int32_t *step) { *step += 1; }
+
+}  // namespace
+
+extern "C" int cmaq_ros3_run_steps_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, int32_t nsteps,
+                                          const double *temp, const double *pres, const double *h2o,
+                                          const double *rj, const double *het, const double *sea, double *y,
+                                          double *y_ckpt, int32_t *nacc, int32_t *nrej, cmaq_ros3_fail *fail,
+                                          void *stream)
+{
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (nsteps < 0 || (n > 0 && (!temp || !pres || !h2o || !y || (c->nphotab > 0 && !rj))))
+        return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0 || nsteps == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->gstream) HIP_TRY(c, hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+    hipStream_t g = c->gstream;
+    if (c->rki_cells < n) {
+        hipFree(c->d_rki);
+        c->d_rki = nullptr;
+        c->rki_cells = 0;
+        if (hipMalloc((void **)&c->d_rki, (size_t)n * c->nr * sizeof(double)) != hipSuccess)
+            return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of rate constants failed");
+        c->rki_cells = n;
+    }
+    if (c->step_cells < n) {
+        hipFree(c->d_step_na); hipFree(c->d_step_nr);
+        c->d_step_na = c->d_step_nr = nullptr;
+        c->step_cells = 0;
+        if (hipMalloc((void **)&c->d_step_na, (size_t)n * 4) != hipSuccess ||
+            hipMalloc((void **)&c->d_step_nr, (size_t)n * 4) != hipSuccess)
+            return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of step counts failed");
+        c->step_cells = n;
+    }
+    int32_t *d_step = (int32_t *)((char *)c->d_small + 48);
+    // (re)capture when anything the graph bakes in has changed
+    SolveParams P;
+    solve_params(c, P, n, chemstep, c->d_rki, y, c->d_step_na, c->d_step_nr);
+    std::vector<const void *> key = {(const void *)(intptr_t)n, temp, pres, h2o, rj, het, sea, y, y_ckpt, nacc,
+                                     nrej, c->mk, c->d_ws, c->d_rki, c->d_step_na,
+                                     (const void *)(intptr_t)P.refill_min, (const void *)(intptr_t)c->grid_blocks};
+    double cs = chemstep;
+    const void *csbits;
+    std::memcpy(&csbits, &cs, sizeof cs);
+    key.push_back(csbits);
+    if (!c->graph_exec || key != c->graph_key) {
+        if (c->graph_exec) { hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+        if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+        HIP_TRY(c, hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal));
+        hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, g, c->d_queue);
+        int rc = launch_rates(c, n, temp, pres, h2o, rj, het, sea, c->d_rki, g);
+        void *args[] = {&P};
+        hipError_t e = hipLaunchKernel(c->mk->solve_kernel, dim3(c->grid_blocks), dim3(kSolveBlock), args, 0, g);
+        CkptParams K{n, c->ns, y, y_ckpt, c->d_step_na, c->d_step_nr, nacc, nrej, d_step};
+        hipLaunchKernelGGL(checkpoint_kernel, dim3(1024), dim3(256), 0, g, K);
+        hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(1), 0, g, d_step);
+        hipGraph_t gr = nullptr;
+        hipError_t e2 = hipStreamEndCapture(g, &gr);
+        if (rc) { if (gr) hipGraphDestroy(gr); return rc; }
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (gr) hipGraphDestroy(gr);
+            return set_err(c, CMAQ_ROS3_E_HIP, std::string("graph capture failed: ") +
+                                                   hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        c->graph = gr;
+        HIP_TRY(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        c->graph_key = key;
+    }
+    // order after the caller's prior work on `stream`
+    HIP_TRY(c, hipEventRecord(c->ev[0], (hipStream_t)stream));
+    HIP_TRY(c, hipStreamWaitEvent(g, c->ev[0], 0));
+    hipLaunchKernelGGL(small_reset_kernel, dim3(1), dim3(64), 0, g, (unsigned long long *)c->d_small);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipEventRecord(c->ev[1], g));
+    for (int32_t k = 0; k < nsteps; ++k) HIP_TRY(c, hipGraphLaunch(c->graph_exec, g));
+    HIP_TRY(c, hipEventRecord(c->ev[2], g));
+    unsigned char small[64];
+    HIP_TRY(c, hipMemcpyAsync(small, c->d_small, 64, hipMemcpyDeviceToHost, g));
+    HIP_TRY(c, hipStreamSynchronize(g));
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+    c->steps_ms = ms;
+    return check_fail(c, small, fail);
+}
+
+extern "C" int cmaq_ros3_last_steps_timing(const cmaq_ros3_ctx *c, double *ms)
+{
+    if (!c || !ms) return CMAQ_ROS3_E_ARG;
+    *ms = c->steps_ms;
+    return 0;
 }

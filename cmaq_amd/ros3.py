@@ -84,6 +84,9 @@ def lib():
         L.cmaq_ros3_solve_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 4 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_adjoint_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 5 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_last_adjoint_timing.argtypes = [vp, _f64p, _f64p, C.POINTER(C.c_int64)]
+        L.cmaq_ros3_run_steps_device.argtypes = [vp, C.c_int64, C.c_double, C.c_int32] + [vp] * 10 + \
+            [C.POINTER(_Fail), vp]
+        L.cmaq_ros3_last_steps_timing.argtypes = [vp, _f64p]
         L.cmaq_ros3_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
         L.cmaq_ros3_species_order.argtypes = [vp, _i32p]
         L.cmaq_ros3_last_error.argtypes = [vp]
@@ -281,6 +284,31 @@ class Ros3Solver:
                 raise ValueError(f"{name} must be {want}")
             if t.numel() != rows * n:
                 raise ValueError(f"{name}: expected {rows} x {n} elements, got {t.numel()}")
+
+    def run_steps_device(self, chemstep, nsteps, temp, pres_atm, h2o_ppm, rj, y, y_ckpt=None, nacc=None,
+                         nrej=None, het=None, seawater=None, stream=None, raise_on_fail=True):
+        """nsteps sync steps through the captured graph; y [ns, n] in place,
+        y_ckpt [nsteps, ns, n], nacc / nrej [nsteps, n] (optional)."""
+        import torch
+        n = temp.numel()
+        self._check([("temp", temp, 1), ("pres", pres_atm, 1), ("h2o", h2o_ppm, 1), ("rj", rj, self.nphotab),
+                     ("y", y, self.ns), ("y_ckpt", y_ckpt, nsteps * self.ns), ("nacc", nacc, nsteps),
+                     ("nrej", nrej, nsteps), ("het", het, self.nhetero), ("sea", seawater, 1)], n)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_ros3_run_steps_device(self._ctx, n, chemstep, nsteps, _ptr(temp), _ptr(pres_atm),
+                                              _ptr(h2o_ppm), _ptr(rj), _ptr(het), _ptr(seawater), _ptr(y),
+                                              _ptr(y_ckpt), _ptr(nacc), _ptr(nrej), C.byref(f), C.c_void_p(stream))
+        fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, fail)
+        return fail if rc != 0 else None
+
+    def last_steps_timing(self):
+        v = C.c_double()
+        lib().cmaq_ros3_last_steps_timing(self._ctx, C.byref(v))
+        return v.value
 
     def solve_device(self, chemstep, rki, y, nacc=None, nrej=None, stream=None, raise_on_fail=True):
         """RBSOLVER with the rate constants given: rki [nr, n], y [ns, n]

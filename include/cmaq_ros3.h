@@ -136,6 +136,28 @@ int cmaq_ros3_run_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min
                          cmaq_ros3_fail *fail, void *stream);
 
 /*
+ * nsteps consecutive chemistry sync steps (CHEM calls, rbdriver.F:20) with
+ * the same met inputs, on device arrays.  Each sync step -- reset of the cell
+ * queue, CALC_RCONST, RBSOLVER over all cells, checkpoint -- is captured once
+ * in a hipGraph and replayed per step; the graph is rebuilt only when the
+ * sizes, pointers or kernel variant change.  y is advanced in place;
+ * y_ckpt [nsteps][numb_mech_spc][ncells] (may be NULL) receives y at the end
+ * of every sync step, n_accepted / n_rejected [nsteps][ncells] (may be NULL)
+ * the per-step counts.  The first failure of any step is reported as in
+ * cmaq_ros3_run_device; the other cells and steps still run.  Ordered after
+ * prior work on `stream`; returns when all steps have finished.
+ */
+int cmaq_ros3_run_steps_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min,
+                               int32_t nsteps, const double *temp_k, const double *pres_atm,
+                               const double *h2o_ppm, const double *rj_per_min,
+                               const double *het_per_min, const double *seawater,
+                               double *y_ppm, double *y_ckpt, int32_t *n_accepted,
+                               int32_t *n_rejected, cmaq_ros3_fail *fail, void *stream);
+
+/* Device time of all sync steps of the last cmaq_ros3_run_steps_device call. */
+int cmaq_ros3_last_steps_timing(const cmaq_ros3_ctx *ctx, double *ms);
+
+/*
  * RBSOLVER alone over ncells cells with the rate constants supplied
  * (the RKI array of rbdata_mod.F, filled by CALC_RCONST at rbdriver.F:521):
  * rki [nrxns][ncells] in ppm-min units, device memory.  Otherwise as

@@ -80,6 +80,8 @@ def lib():
         L.om_replay_cell.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, C.c_int]
         L.om_adjoint_cell.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, C.c_int, _f64p, _f64p]
         L.om_adjoint_run.argtypes = [vp, C.c_int64, C.c_double] + [_f64p] * 7 + [_i32p]
+        L.om_ddm_cell.argtypes = [vp, C.c_double, _f64p, _f64p, _f64p, _f64p, C.c_int, _i32p,
+                                  C.POINTER(C.c_float)]
         _lib = L
     return _lib
 
@@ -293,3 +295,36 @@ class Oracle:
                                   _p(at, _f64p), _p(lam, _f64p), _p(gk, _f64p), _p(yend, _f64p),
                                   _p(na, _i32p))
         return {"rc": rc, "lam": lam, "gk": gk, "y": yend, "nacc": na}
+
+    # ---- CMAQ-DDM-3D first-order chemistry sensitivities (one cell) ----
+    def ddm_cell(self, chemstep, rki, y0, nsteps=1, rtol=1e-3, atol=1e-7, w=None):
+        """Sensitivities of y_end to ln C0 of every species and to ln k of every
+        reaction over nsteps sync steps, as DDM-3D computes them.  Returns
+        (S_init [ns(param), ns(species)], S_rate [nr, ns], y_end), mechanism
+        order; with a seed w, returns the contractions w.S instead of S."""
+        ns, nr = self.ns, self.nr
+        inew = self.inew2old() - 1
+        rk = np.ascontiguousarray(rki, np.float64)
+        y = self._to_new(y0)
+        rt = self._to_new(np.broadcast_to(rtol, (ns,)))
+        at = self._to_new(np.broadcast_to(atol, (ns,)))
+        npar = ns + nr
+        prate = np.zeros(npar, np.int32)
+        prate[ns:] = np.arange(1, nr + 1)
+        sens = np.zeros((npar, ns), np.float32)
+        # INIT parameter j (mechanism species j): S0 = C0 at species j (new order)
+        y0m = np.asarray(y0, np.float64)
+        for j in range(ns):
+            sens[j, np.flatnonzero(inew == j)[0]] = np.float32(y0m[j])
+        for _ in range(nsteps):
+            rc = lib().om_ddm_cell(self._ctx, chemstep, _p(rk, _f64p), _p(y, _f64p), _p(rt, _f64p), _p(at, _f64p),
+                                   npar, _p(prate, _i32p), sens.ctypes.data_as(C.POINTER(C.c_float)))
+            if rc != 0:
+                raise RuntimeError(f"om_ddm_cell failed ({rc})")
+        s_old = np.empty_like(sens)
+        s_old[:, inew] = sens
+        yend = self._to_old(y)
+        if w is None:
+            return s_old[:ns].astype(np.float64), s_old[ns:].astype(np.float64), yend
+        wv = np.asarray(w, np.float64)
+        return s_old[:ns].astype(np.float64) @ wv, s_old[ns:].astype(np.float64) @ wv, yend

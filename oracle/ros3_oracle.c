@@ -1020,3 +1020,163 @@ int om_adjoint_run(const om_ctx *c, int64_t ncells, double chemstep, const doubl
     free(h); free(y); free(ys); free(l); free(rk); free(g); free(rt); free(at);
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* CMAQ-DDM-3D first-order chemistry sensitivities                     */
+/* ------------------------------------------------------------------ */
+/* Restates the DDM-3D chemistry step as the ros3 build with -Dsens runs it:
+ * the forward step accumulates the time-averaged concentration
+ * YAVE = sum over accepted steps of (Y + max(YNEW, CONMIN)) / 2 * DT, divided
+ * by CHEMSTEP (rbsolver.F:341-343, 468-472, 592-596); then, per parameter,
+ * one Crank-Nicolson step over the whole sync step with the Jacobian at YAVE
+ * (ddm3d/DDM3D_CHEM.F:296-395): A = I - h/2 J, A1 = I + h/2 J in REAL(4),
+ * A S_new = A1 S_old + P_rate, LINPACK SGEFA/SGESL (ddm3d/s_lu.F).  Rate
+ * parameters (IPT = 5) add P_rate = h k p(YAVE) S_r (DDM3D_CHEM.F:340-384),
+ * i.e. sensitivities to ln k; INIT parameters are seeded with S0 = C0
+ * (init/initscen.F:306-314), i.e. sensitivities to ln C0. */
+
+/* LINPACK SGEFA: Gaussian elimination with partial pivoting, column major. */
+static int sgefa(float *a, int n, int *ipvt)
+{
+    int info = 0;
+    for (int k = 0; k < n - 1; ++k) {
+        int l = k;
+        float m = fabsf(a[k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabsf(a[k * n + i]) > m) { m = fabsf(a[k * n + i]); l = i; }
+        ipvt[k] = l;
+        if (a[k * n + l] == 0.0f) { info = k + 1; continue; }
+        if (l != k) { float t = a[k * n + l]; a[k * n + l] = a[k * n + k]; a[k * n + k] = t; }
+        float t = -1.0f / a[k * n + k];
+        for (int i = k + 1; i < n; ++i) a[k * n + i] *= t;
+        for (int j = k + 1; j < n; ++j) {
+            float tj = a[j * n + l];
+            if (l != k) { a[j * n + l] = a[j * n + k]; a[j * n + k] = tj; }
+            for (int i = k + 1; i < n; ++i) a[j * n + i] += tj * a[k * n + i];
+        }
+    }
+    ipvt[n - 1] = n - 1;
+    if (a[(n - 1) * n + n - 1] == 0.0f) info = n;
+    return info;
+}
+
+/* LINPACK SGESL with job = 0: solves A x = b in place. */
+static void sgesl(const float *a, int n, const int *ipvt, float *b)
+{
+    for (int k = 0; k < n - 1; ++k) {
+        int l = ipvt[k];
+        float t = b[l];
+        if (l != k) { b[l] = b[k]; b[k] = t; }
+        for (int i = k + 1; i < n; ++i) b[i] += t * a[k * n + i];
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        b[k] /= a[k * n + k];
+        float t = -b[k];
+        for (int i = 0; i < k; ++i) b[i] += t * a[k * n + i];
+    }
+}
+
+int om_ddm_cell(const om_ctx *c, double chemstep, const double *rki, double *y,
+                const double *rtol, const double *atol, int np, const int32_t *prate, float *sens)
+{
+    const int ns = c->ns, ncsp = 1;
+    const double rnspec = (double)(1.0f / (float)ns);
+    om_work w;
+    work_alloc(c, ncsp, &w);
+    double *yave = xcalloc(ns, 8);
+    double dt = DTMIN > DTSTART ? DTMIN : DTSTART, tnow = 0.0;
+    int lpass = 0, rc = 0;
+    long attempts = 0;
+    while (tnow < chemstep) {
+        if (++attempts > OM_MAX_ATTEMPTS) { rc = -2; break; }
+        double tend = tnow + dt;
+        if (tend > chemstep) { dt = chemstep - tnow; tend = chemstep; }
+        ros3_stages1(c, ncsp, rki, y, dt, w.ydot, w.cc2, w.vdiag, w.k1, w.k2, w.k3, w.yp, w.yn);
+        double err = 0.0;
+        for (int n = 0; n < ns; ++n) {
+            double ytol = atol[n] + rtol[n] * fabs(w.yn[n]);
+            double e = (D1 * w.k1[n] + D2 * w.k2[n] + D3 * w.k3[n]) / ytol;
+            err = err + e * e;
+        }
+        double s = sqrt(err * rnspec), maxerr = UROUND;
+        if (!(s <= maxerr)) maxerr = s;
+        double dtfac = 0.9 / pow(maxerr, GROW);
+        if (maxerr <= 1.0) {
+            for (int q = 0; q < ns; ++q) {
+                double yn = w.yn[q] > CONMIN ? w.yn[q] : CONMIN;
+                yave[q] = yave[q] + 0.5 * (y[q] + yn) * dt;
+                y[q] = yn;
+            }
+            tnow = tend;
+            double hi = lpass ? FACMAX : FACONE;
+            if (dtfac > hi) dtfac = hi;
+            if (dtfac < FACMIN) dtfac = FACMIN;
+            double dn = dtfac * dt;
+            if (dn < DTMIN) dn = DTMIN;
+            dt = dn < DTMAX ? dn : DTMAX;
+            lpass = 1;
+        } else {
+            dt = FACMIN * dt;
+            lpass = 0;
+            if (dt < DTMIN) { rc = -1; break; }
+        }
+    }
+    work_free(&w);
+    if (rc) { free(yave); return rc; }
+    for (int q = 0; q < ns; ++q) yave[q] = yave[q] / chemstep;
+    /* Jacobian at YAVE accumulated in REAL(4) (MECHANISM_FUNCTIONS.F:831-869) */
+    float *jac = xcalloc((size_t)ns * ns, 4), *a = xcalloc((size_t)ns * ns, 4), *a1 = xcalloc((size_t)ns * ns, 4);
+    for (int nrx = 1; nrx <= c->nuserat[ncsp]; ++nrx) {
+        int r = c->nkuserat[ncsp][nrx], nre = c->t.nreact[r - 1];
+        for (int q = 1; q <= nre; ++q) {
+            int col = IRM2(c, r, q) - 1;
+            double ex = rki[r - 1] * pexcept(c, r, yave, q, 0);
+            for (int q2 = 1; q2 <= nre; ++q2) {
+                int row = IRM2(c, r, q2) - 1;
+                jac[col * ns + row] += (float)(-1.0 * ex);
+            }
+            for (int p = 1; p <= c->t.nprdct[r - 1]; ++p) {
+                int row = IRM2(c, r, p + 3) - 1;
+                jac[col * ns + row] += (float)(SC(c, r, p) * ex);
+            }
+        }
+    }
+    for (int j = 0; j < ns; ++j)
+        for (int i = 0; i < ns; ++i) {
+            a[j * ns + i] = (float)(-0.5 * chemstep * (double)jac[j * ns + i]);
+            a1[j * ns + i] = (float)(0.5 * chemstep * (double)jac[j * ns + i]);
+            if (i == j) { a[j * ns + i] = 1.0f + a[j * ns + i]; a1[j * ns + i] = 1.0f + a1[j * ns + i]; }
+        }
+    int *ipvt = xcalloc(ns, sizeof(int));
+    sgefa(a, ns, ipvt);
+    float *sold = xcalloc(ns, 4), *prd = xcalloc(ns, 4), *prate_v = xcalloc(ns, 4);
+    for (int p = 0; p < np; ++p) {
+        float *sp = sens + (size_t)p * ns;
+        for (int q = 0; q < ns; ++q) {
+            sold[q] = fabsf(sp[q]) < 1.0e-25f ? 0.0f : sp[q];
+            prate_v[q] = 0.0f;
+        }
+        int r = prate ? prate[p] : 0;
+        if (r > 0) {
+            float kstep = (float)(rki[r - 1] * chemstep);
+            float rxm = (float)((double)kstep * pexcept(c, r, yave, 0, 0));
+            for (int q = 1; q <= c->t.nreact[r - 1]; ++q) {
+                int s = IRM2(c, r, q) - 1;
+                prate_v[s] = prate_v[s] - rxm * 1.0f;
+            }
+            for (int q = 1; q <= c->t.nprdct[r - 1]; ++q) {
+                int s = IRM2(c, r, q + 3) - 1;
+                prate_v[s] = (float)((double)prate_v[s] + (double)rxm * SC(c, r, q) * 1.0);
+            }
+        }
+        for (int s = 0; s < ns; ++s) {
+            double total = 0.0;
+            for (int j = 0; j < ns; ++j) total = total + (double)(a1[j * ns + s] * sold[j]);
+            prd[s] = (float)(total + (double)prate_v[s]);
+        }
+        sgesl(a, ns, ipvt, prd);
+        for (int s = 0; s < ns; ++s) sp[s] = fabsf(prd[s]) < 1.0e-25f ? 0.0f : prd[s];
+    }
+    free(yave); free(jac); free(a); free(a1); free(ipvt); free(sold); free(prd); free(prate_v);
+    return 0;
+}

@@ -116,6 +116,15 @@ int om_adjoint_run(const om_ctx *c, int64_t ncells, double chemstep, const doubl
                    const double *y0, const double *rtol, const double *atol,
                    double *lam, double *gk, double *yend, int32_t *nacc);
 
+/* CMAQ-DDM-3D chemistry step (ddm3d/DDM3D_CHEM.F:155-511) for one cell and
+ * one sync step: advances y (NEW order) with per-cell RBSOLVER while
+ * accumulating YAVE, then applies the REAL(4) Crank-Nicolson sensitivity
+ * step to each of the np parameters.  sens [np][ns] (NEW order) in/out;
+ * prate[p] = 1-based reaction of a rate parameter, 0 for an initial-
+ * condition parameter (or prate = NULL). */
+int om_ddm_cell(const om_ctx *c, double chemstep, const double *rki, double *y,
+                const double *rtol, const double *atol, int np, const int32_t *prate, float *sens);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,11 +38,11 @@ class Programs:
     """The forward and adjoint programs of one mechanism as Python functions
     on a workspace dict of lists (species in the solver's order)."""
 
-    def __init__(self, sym):
+    def __init__(self, sym, jac_order="column"):
         self.sym = sym
         fe = P.feval_ops(sym, "scatter")
         self.feval = compile_ops(fe)
-        self.jac_lu = compile_ops(P.jac_lu_ops(sym, "column"), ["gdtinv"])
+        self.jac_lu = compile_ops(P.jac_lu_ops(sym, jac_order), ["gdtinv"])
         self.solve = [compile_ops(P.solve_ops(sym, s), ["x1", "x2"]) for s in (1, 2, 3)]
         self.yp = compile_ops(P.yp_ops(sym))
         self.seed = compile_ops(P.adj_seed_ops(sym))
@@ -75,9 +75,11 @@ class Programs:
 
     def ynew(self, W):
         c = CONST
-        return [max(((y + c["B1"] * a) + c["B2"] * b) + c["B3"] * d, c["CONMIN"]) if
-                ((y + c["B1"] * a) + c["B2"] * b) + c["B3"] * d > c["CONMIN"] else c["CONMIN"]
-                for y, a, b, d in zip(W["Y"], W["K1"], W["K2"], W["K3"])]
+        out = []
+        for y, a, b, d in zip(W["Y"], W["K1"], W["K2"], W["K3"]):
+            v = ((y + c["B1"] * a) + c["B2"] * b) + c["B3"] * d
+            out.append(v if v > c["CONMIN"] else c["CONMIN"])
+        return out
 
     def adjoint_step(self, W, h):
         """W holds Y (step start), LAM (dL/dYnew); on exit LAM = dL/dY, GK += dL/dk."""

@@ -96,3 +96,17 @@ def test_reserved_rate_types_rejected(cb6r5):
     m["ktype"][r] = 13
     rc, msg = _init_rc(L, m)
     assert rc == -2 and "type" in msg.lower(), msg
+
+
+def test_python_binding_matches_header():
+    """Every function the ctypes binding declares takes as many arguments as
+    its prototype in include/cmaq_ros3.h."""
+    L = _lib()
+    text = open(HEADER).read()
+    decls = re.findall(r"\n(?:int|void|const char \*)\s*(cmaq_ros3_\w+)\s*\(([^;]*?)\);", text, re.S)
+    assert len(decls) == len(_declared())
+    for name, params in decls:
+        n = 0 if params.strip() in ("", "void") else params.count(",") + 1
+        at = getattr(L, name).argtypes
+        assert at is not None, name
+        assert len(at) == n, (name, len(at), n)

@@ -218,3 +218,38 @@ def test_consecutive_sync_steps(cb6r5, oracle_cb6r5, solver_cb6r5):
         yg = y.cpu().numpy()
         assert ((np.abs(yg - yr) > tol)[:, ok]).sum() == 0
         yr = yg.copy()
+
+
+def test_graph_sync_loop_matches_eager(cb6r5, solver_cb6r5):
+    """The hipGraph sync-step loop equals the same steps run eagerly, bit for
+    bit, on its first capture and on a replay of the same graph after other
+    eager work (checkpoints and per-step counts included)."""
+    import torch
+    n, k = 2000, 3
+    d = make_cells(cb6r5, np.arange(n), seed=31, workload="mixed")
+    t = {f: _dev(d[f]) for f in ("temp", "pres", "h2o", "rj")}
+    ye = _dev(d["y"]).clone()
+    ref_y, ref_na = [], []
+    for _ in range(k):
+        na = torch.zeros(n, dtype=torch.int32, device="cuda")
+        solver_cb6r5.run_device(5.0, t["temp"], t["pres"], t["h2o"], t["rj"], ye, na, raise_on_fail=False)
+        ref_y.append(ye.clone())
+        ref_na.append(na.clone())
+    yg = torch.empty_like(ye)
+    ck = torch.empty((k,) + tuple(ye.shape), dtype=torch.float64, device="cuda")
+    nas = torch.empty((k, n), dtype=torch.int32, device="cuda")
+    for rep in range(2):
+        yg.copy_(_dev(d["y"]))
+        ck.zero_()
+        nas.zero_()
+        solver_cb6r5.run_steps_device(5.0, k, t["temp"], t["pres"], t["h2o"], t["rj"], yg, ck, nas,
+                                      raise_on_fail=False)
+        torch.cuda.synchronize()
+        for s in range(k):
+            assert torch.equal(ck[s], ref_y[s]), (rep, s)
+            assert torch.equal(nas[s], ref_na[s]), (rep, s)
+        assert torch.equal(yg, ref_y[-1])
+        # eager work between the replays
+        yx = _dev(d["y"]).clone()
+        solver_cb6r5.run_device(5.0, t["temp"], t["pres"], t["h2o"], t["rj"], yx, raise_on_fail=False)
+        torch.cuda.synchronize()

@@ -20,8 +20,11 @@ def _cell(mech, o, seed):
     return rki[:, 0].copy(), d["y"][:, 0].copy()
 
 
-def test_forward_programs_bit_exact(cb6r5, oracle_cb6r5, prog_cb6r5):
-    o, pg = oracle_cb6r5, prog_cb6r5
+@pytest.mark.parametrize("jac_order", ["column", "row", "group3", "tile10x10", "tile6x10"])
+def test_forward_programs_bit_exact(cb6r5, oracle_cb6r5, jac_order):
+    """Every LU schedule keeps each entry's reference operation sequence, so
+    all reproduce the oracle bit for bit."""
+    o, pg = oracle_cb6r5, Programs(analyse(cb6r5), jac_order)
     inew = o.inew2old() - 1
     for seed in (1, 4):
         rki, y0 = _cell(cb6r5, o, seed)
