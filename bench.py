@@ -36,6 +36,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 FP64_PEAK_TFS = 78.6       # MI355X FP64 vector, TFLOP/s
 
 
@@ -53,6 +54,9 @@ def _args():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--adj-cells", type=int, default=262144)
     p.add_argument("--adj-sync-steps", type=int, default=12)
+    p.add_argument("--adj-mech", default=None, help="mechanism of the adjoint leg (default --mech)")
+    p.add_argument("--adj-svoc", action="store_true",
+                   help="adjoint leg: VOC/SVOC species log-uniform on [1e-5, 5] ppb (the CRACMM2 workload)")
     p.add_argument("--no-adj", action="store_true")
     p.add_argument("--loop-cells", type=int, default=1048576)
     p.add_argument("--loop-sync-steps", type=int, default=12)
@@ -105,6 +109,21 @@ def cpu_baseline(args, mech, ncells, nlay_cells):
             "failed_blocks": int(sum(1 for r in res if r[3] != 0))}
 
 
+def measured_traffic_per_step(mech_name, variant):
+    """HBM bytes per attempted cell-step of the solver kernel, from the
+    calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE summary under profiles/, or
+    None when no summary covers this mechanism and kernel variant."""
+    import glob
+    import re
+    key = re.escape(f"[{mech_name}/{variant}]")
+    for path in sorted(glob.glob(os.path.join(PROFILES, "*_pmc.md"))):
+        with open(path) as f:
+            m = re.search(r"traffic_bytes_per_attempted_cell_step" + key + r"\s*=\s*([0-9.eE+]+)", f.read())
+        if m:
+            return float(m.group(1)), os.path.basename(path)
+    return None, None
+
+
 def loop_leg(args, mech, solver, dev):
     """12 sync steps of the mixed workload through the captured graph."""
     import torch
@@ -141,8 +160,13 @@ def adjoint_leg(args, mech, solver, dev):
     import torch
     from cmaq_amd.adjoint import integrate
     from cmaq_amd.inputs import make_cells_torch
+    from cmaq_amd.ros3 import Ros3Solver, load_mechanism
+    if args.adj_mech and args.adj_mech != args.mech:
+        mech = load_mechanism(args.adj_mech)
+        solver = Ros3Solver(mech)
     n, k = args.adj_cells, args.adj_sync_steps
-    d = make_cells_torch(mech, n, seed=args.seed + 3, workload="mixed", device=dev)
+    d = make_cells_torch(mech, n, seed=args.seed + 3, workload="mixed", device=dev,
+                         svoc_ppb=(1e-5, 5.0) if args.adj_svoc else None)
     rki = torch.empty((mech["nrxns"], n), dtype=torch.float64, device=dev)
     solver.rates_device(d["temp"], d["pres"], d["h2o"], d["rj"], rki)
     names = mech["chemistry_spc"]
@@ -182,10 +206,11 @@ def adjoint_leg(args, mech, solver, dev):
     print(f"[bench] adjoint leg: forward+adjoint {t_both * 1e3:.0f} ms", file=sys.stderr, flush=True)
     bad = int((~torch.isfinite(gy)).any(dim=0).sum())
     return t_both / t_fwd, {
-        "workload": f"{args.mech} ROS3, mixed regimes, {n} cells, {k} sync steps of 5 min; "
+        "workload": f"{mech['mechname'].lower()} ROS3, mixed regimes{' + VOC/SVOC' if args.adj_svoc else ''}, "
+                    f"{n} cells, {k} sync steps of 5 min; "
                     f"loss = sum over cells of final O3 + 0.1 NO2; gradients w.r.t. all initial "
                     f"concentrations and all rate constants; sync-step states stored, stages recomputed",
-        "ncells": n, "sync_steps": k, "fwd_ms": t_fwd * 1e3, "fwd_adj_ms": t_both * 1e3,
+        "ncells": n, "sync_steps": k, "variant": solver.variant, "fwd_ms": t_fwd * 1e3, "fwd_adj_ms": t_both * 1e3,
         "accepted_cell_steps": accepted,
         "last_sync_step_replay_ms": rep, "last_sync_step_reverse_ms": rev, "tape_chunks": chunks,
         "cells_without_gradient": bad}
@@ -261,6 +286,8 @@ def main():
     alg_bytes = float((att * b_step).sum()) + float(ncells * b_io)
     solve_s = float(np.mean(solve_ms)) / 1000.0
     achieved = alg_bytes / solve_s / 1e9
+    per_step, traffic_src = measured_traffic_per_step(args.mech, solver.variant)
+    traffic = per_step * attempted if per_step else None
     value = attempted / (ms_per_step / 1000.0)
 
     out = {
@@ -285,8 +312,9 @@ def main():
         "solve_kernel_ms": float(np.mean(solve_ms)),
         "rates_kernel_ms": float(np.mean(rates_ms)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "solve_kernel", "bytes_per_launch": alg_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src, "kernel": "solve_kernel", "variant": solver.variant,
+                     "bytes_per_launch": alg_bytes},
         "fwd_adj_ratio": None,
     }
     if not (args.no_adj and args.no_loop):

@@ -52,6 +52,8 @@ struct cmaq_ros3_ctx {
     RateMul *d_mul = nullptr;
     double *d_rtol = nullptr, *d_atol = nullptr;
     double *d_ws = nullptr;
+    size_t ws_doubles = 0;    // allocated workspace
+    int64_t ws_stride = 0;    // doubles per wave
     unsigned long long *d_queue = nullptr;   // [0] queue, then fail record
     int32_t *d_fail_flag = nullptr, *d_fail_code = nullptr;
     int64_t *d_fail_cell = nullptr;
@@ -334,19 +336,26 @@ static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
     if (int rc = grid_for(c, mk->reverse_kernel, &grid_b)) return rc;
     const int wpb = kSolveBlock / kWave;
     const int64_t waves = (int64_t)std::max(grid, std::max(grid_a, grid_b)) * wpb;
-    const size_t need = (size_t)waves * mk->nws * kWave;
-    const size_t have = c->mk && c->d_ws ? (size_t)c->waves * c->mk->nws * kWave : 0;
-    if (need > have) {
+    // per-wave workspace stride: NWS rows of 64 doubles plus an optional pad
+    // of rows that staggers the waves' bases across memory channels
+    int64_t pad = 0;
+    if (const char *e = std::getenv("CMAQ_ROS3_WS_PAD")) pad = std::max(0ll, std::atoll(e));
+    const int64_t stride = ((int64_t)mk->nws + pad) * kWave;
+    const size_t need = (size_t)waves * stride;
+    if (need > c->ws_doubles) {
         hipFree(c->d_ws);
         c->d_ws = nullptr;
+        c->ws_doubles = 0;
         if (hipMalloc((void **)&c->d_ws, need * sizeof(double)) != hipSuccess)
             return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of the solver workspace failed");
+        c->ws_doubles = need;
     }
     c->mk = mk;
     c->grid_blocks = grid;
     c->grid_replay = grid_a;
     c->grid_reverse = grid_b;
-    c->waves = std::max<int64_t>(waves, have / ((size_t)mk->nws * kWave));
+    c->ws_stride = stride;
+    c->waves = (int64_t)(c->ws_doubles / (size_t)stride);
     return 0;
 }
 
@@ -556,7 +565,7 @@ static void solve_params(cmaq_ros3_ctx *c, SolveParams &P, int64_t n, double che
     std::memset(&P, 0, sizeof P);
     P.ncells = n; P.cell0 = 0; P.stride = n; P.chemstep = chemstep; P.rki = rki; P.y = y;
     P.nacc = nacc; P.nrej = nrej;
-    P.rtol = c->d_rtol; P.atol = c->d_atol; P.ws = c->d_ws; P.queue = c->d_queue;
+    P.rtol = c->d_rtol; P.atol = c->d_atol; P.ws = c->d_ws; P.ws_stride = c->ws_stride; P.queue = c->d_queue;
     P.fail_flag = c->d_fail_flag; P.fail_cell = c->d_fail_cell; P.fail_info = c->d_fail_info;
     P.fail_code = c->d_fail_code; P.max_attempts = c->max_attempts;
     P.refill_min = c->refill_min > 0 ? c->refill_min : (c->mk->refill_min > 0 ? c->mk->refill_min : 2);
@@ -876,7 +885,7 @@ extern "C" int cmaq_ros3_run_steps_device(cmaq_ros3_ctx *c, int64_t n, double ch
     SolveParams P;
     solve_params(c, P, n, chemstep, c->d_rki, y, c->d_step_na, c->d_step_nr);
     std::vector<const void *> key = {(const void *)(intptr_t)n, temp, pres, h2o, rj, het, sea, y, y_ckpt, nacc,
-                                     nrej, c->mk, c->d_ws, c->d_rki, c->d_step_na,
+                                     nrej, c->mk, c->d_ws, (const void *)(intptr_t)c->ws_stride, c->d_rki, c->d_step_na,
                                      (const void *)(intptr_t)P.refill_min, (const void *)(intptr_t)c->grid_blocks};
     double cs = chemstep;
     const void *csbits;

@@ -238,7 +238,8 @@ struct SolveParams {
     double *y;                // [ns][stride] mechanism order, in/out
     int32_t *nacc, *nrej;     // [stride] or nullptr
     const double *rtol, *atol;// [ns] solver (new) order
-    double *ws;               // workspace, waves * NWS * 64 doubles
+    double *ws;               // workspace: wave w's rows start at ws + w * ws_stride
+    int64_t ws_stride;        // doubles per wave (>= NWS * 64)
     unsigned long long *queue;// next unassigned cell
     int32_t *fail_flag;       // 0 until the first failure
     int64_t *fail_cell;
