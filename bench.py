@@ -288,6 +288,10 @@ def main():
     achieved = alg_bytes / solve_s / 1e9
     per_step, traffic_src = measured_traffic_per_step(args.mech, solver.variant)
     traffic = per_step * attempted if per_step else None
+    from cmaq_amd.codegen.programs import flops_per_step
+    from cmaq_amd.codegen.symbolic import analyse
+    f_step = flops_per_step(analyse(mech))
+    fp64_tfs = f_step * attempted / solve_s / 1e12
     value = attempted / (ms_per_step / 1000.0)
 
     out = {
@@ -315,6 +319,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src, "kernel": "solve_kernel", "variant": solver.variant,
                      "bytes_per_launch": alg_bytes},
+        "fp64": {"achieved": fp64_tfs, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                 "frac": fp64_tfs / FP64_PEAK_TFS, "flops_per_attempted_step": f_step},
         "fwd_adj_ratio": None,
     }
     if not (args.no_adj and args.no_loop):

@@ -18,6 +18,8 @@ the sequence the reference applies to it:
              x2 K2 (rbsolver.F:386-422) is formed in the row's first statement.
 """
 
+import re
+
 from .regalloc import Op
 
 
@@ -400,3 +402,30 @@ def adj_b_ops(sym):
                           [("YB", me), rk, s1, ("Y", other), t], True))
     ops += [Op("{0} = {1};", [("LAM", i), ("YB", i)], False) for i in range(sym.ns)]
     return ops
+
+
+# ---------------------------------------------------------------------------
+# Arithmetic count of one attempted ROS3 step (the FP64 roofline's numerator)
+# ---------------------------------------------------------------------------
+_TOK = re.compile(r"\{\d+\}|[A-Za-z_][A-Za-z0-9_]*|\d+\.\d*(?:e[-+]?\d+)?|[-+*/()]")
+
+
+def op_flops(ops):
+    """Binary floating-point operations (+ - * /) in a list of ops; unary
+    minus is free (a sign flip folded into the instruction)."""
+    n = 0
+    for op in ops:
+        prev = None
+        for t in _TOK.findall(op.fmt.split("=", 1)[1]):
+            if t in "*/+" or (t == "-" and prev is not None and prev not in "(*/+-"):
+                n += 1
+            prev = t
+    return n
+
+
+def flops_per_step(sym):
+    """FP64 operations of one attempted step: two f evaluations, Jacobian and
+    LU, three stage solves, YP = Y + K1, and the new state and error norm of
+    rbsolver.F:425-441 (13 per species) plus sqrt and the controller."""
+    return (2 * op_flops(feval_ops(sym)) + op_flops(jac_lu_ops(sym)) +
+            sum(op_flops(solve_ops(sym, k)) for k in (1, 2, 3)) + sym.ns + 13 * sym.ns + 2)

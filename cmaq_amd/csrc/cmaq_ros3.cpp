@@ -65,6 +65,7 @@ struct cmaq_ros3_ctx {
     double *d_tape = nullptr;
     size_t tape_bytes = 0;
     int64_t *d_tape_off = nullptr;
+    int32_t *d_order = nullptr;       // adjoint queue order, longest cells first
     int64_t tape_off_cells = 0;
     double replay_ms = 0.0, reverse_ms = 0.0;
     int64_t adj_chunks = 0;
@@ -461,7 +462,7 @@ extern "C" void cmaq_ros3_free(cmaq_ros3_ctx *c)
     hipSetDevice(c->device);
     hipFree(c->d_ops); hipFree(c->d_mul); hipFree(c->d_rtol); hipFree(c->d_atol);
     hipFree(c->d_ws); hipFree(c->d_small); hipFree(c->d_rki); hipFree(c->d_io);
-    hipFree(c->d_tape); hipFree(c->d_tape_off);
+    hipFree(c->d_tape); hipFree(c->d_tape_off); hipFree(c->d_order);
     hipFree(c->d_step_na); hipFree(c->d_step_nr);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
@@ -679,27 +680,31 @@ extern "C" int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *c, int64_t n, double chem
     HIP_TRY(c, hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
     const int rs = c->mk->tape_rs;
-    // per-cell tape offsets from the forward's accepted-step counts; the cells
-    // are split into chunks whose tapes fit the tape budget
+    // per-cell tape offsets from the forward's accepted-step counts.  Cells
+    // are queued longest first (so no long cell starts near the end of a
+    // launch) and split into chunks whose tapes fit the tape budget.
     std::vector<int32_t> na((size_t)n);
     HIP_TRY(c, hipMemcpyAsync(na.data(), nacc_fwd, (size_t)n * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
     size_t free_b = 0, total_b = 0;
     HIP_TRY(c, hipMemGetInfo(&free_b, &total_b));
-    size_t budget = (size_t)((double)(free_b + c->tape_bytes) * 0.5);
+    size_t budget = (size_t)((double)(free_b + c->tape_bytes) * 0.8);
     if (const char *e = std::getenv("CMAQ_ROS3_TAPE_BYTES")) budget = std::min<size_t>(budget, std::strtoull(e, nullptr, 10));
     const int64_t budget_rec = (int64_t)(budget / ((size_t)rs * sizeof(double)));
+    std::vector<int32_t> order((size_t)n);
+    for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return na[a] > na[b]; });
     std::vector<int64_t> off((size_t)n);
-    std::vector<int64_t> chunk_beg{0}, chunk_rec;
+    std::vector<int64_t> chunk_beg{0};
     int64_t acc = 0, max_rec = 0;
-    for (int64_t i = 0; i < n; ++i) {
+    for (int64_t q = 0; q < n; ++q) {
+        const int32_t i = order[q];
         const int64_t k = na[i] > 0 ? na[i] : 0;
         if (k > budget_rec) return set_err(c, CMAQ_ROS3_E_NOMEM, "one cell's tape exceeds the tape budget");
-        if (acc + k > budget_rec) { chunk_rec.push_back(acc); chunk_beg.push_back(i); max_rec = std::max(max_rec, acc); acc = 0; }
+        if (acc + k > budget_rec) { chunk_beg.push_back(q); max_rec = std::max(max_rec, acc); acc = 0; }
         off[i] = acc;
         acc += k;
     }
-    chunk_rec.push_back(acc);
     max_rec = std::max(max_rec, acc);
     const size_t need = (size_t)std::max<int64_t>(max_rec, 1) * rs * sizeof(double);
     if (c->tape_bytes < need) {
@@ -711,13 +716,17 @@ extern "C" int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *c, int64_t n, double chem
     }
     if (c->tape_off_cells < n) {
         hipFree(c->d_tape_off);
+        hipFree(c->d_order);
         c->d_tape_off = nullptr;
+        c->d_order = nullptr;
         c->tape_off_cells = 0;
-        if (hipMalloc((void **)&c->d_tape_off, (size_t)n * sizeof(int64_t)) != hipSuccess)
+        if (hipMalloc((void **)&c->d_tape_off, (size_t)n * sizeof(int64_t)) != hipSuccess ||
+            hipMalloc((void **)&c->d_order, (size_t)n * sizeof(int32_t)) != hipSuccess)
             return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of tape offsets failed");
         c->tape_off_cells = n;
     }
     HIP_TRY(c, hipMemcpyAsync(c->d_tape_off, off.data(), (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->d_order, order.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, s));
     SolveParams P;
     solve_params(c, P, n, chemstep, rki, const_cast<double *>(y0), nullptr, nullptr);
     P.tape = c->d_tape; P.tape_off = c->d_tape_off; P.nacc_fwd = nacc_fwd; P.lam = lam; P.grad_rki = grad_rki;
@@ -726,7 +735,8 @@ extern "C" int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *c, int64_t n, double chem
     c->adj_chunks = (int64_t)chunk_beg.size();
     for (size_t k = 0; k < chunk_beg.size(); ++k) {
         const int64_t b = chunk_beg[k], e = k + 1 < chunk_beg.size() ? chunk_beg[k + 1] : n;
-        P.cell0 = b;
+        P.order = c->d_order + b;
+        P.cell0 = 0;
         P.ncells = e - b;
         void *args[] = {&P};
         HIP_TRY(c, hipMemsetAsync(c->d_queue, 0, 8, s));
@@ -746,6 +756,11 @@ extern "C" int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *c, int64_t n, double chem
     unsigned char small[64];
     HIP_TRY(c, hipMemcpyAsync(small, c->d_small, 64, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
+    if (c->tape_bytes > ((size_t)8 << 30)) {   // do not hold most of the device between calls
+        hipFree(c->d_tape);
+        c->d_tape = nullptr;
+        c->tape_bytes = 0;
+    }
     return check_fail(c, small, fail);
 }
 

@@ -230,8 +230,9 @@ __global__ __launch_bounds__(256) void rates_kernel(RatesParams P)
 
 // ---- persistent integration kernel ------------------------------------
 struct SolveParams {
-    int64_t ncells;           // cells of this launch: cell0 .. cell0 + ncells - 1
-    int64_t cell0;
+    int64_t ncells;           // cells of this launch: cell0 .. cell0 + ncells - 1,
+    int64_t cell0;            // or order[0 .. ncells - 1] when order is set
+    const int32_t *order;     // optional queue order (cell indices), else nullptr
     int64_t stride;           // leading dimension of the cell arrays
     double chemstep;
     const double *rki;        // [nr][stride]

@@ -176,9 +176,10 @@ int cmaq_ros3_solve_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_m
  * order), overwrites lambda with dL/dy0 and adds dL/drki to grad_rki
  * [nrxns][ncells] (NULL skips it).  The accepted step sizes are held fixed
  * (the step-size controller is not differentiated).  The forward is replayed
- * cell by cell into per-step tapes, in chunks of cells that fit half of the
- * free device memory (CMAQ_ROS3_TAPE_BYTES lowers the budget); a replay that
- * does not reproduce n_accepted returns CMAQ_ROS3_E_TAPE.  Cells with
+ * cell by cell into per-step tapes, longest cells first, in chunks of cells
+ * whose tapes fit 80 % of the free device memory (CMAQ_ROS3_TAPE_BYTES lowers
+ * the budget; tapes above 8 GiB are released when the call returns); a
+ * replay that does not reproduce n_accepted returns CMAQ_ROS3_E_TAPE.  Cells with
  * n_accepted < 0 (failed forward) get NaN in lambda.  Device memory, `stream`
  * as in cmaq_ros3_run_device; returns after the stream has finished.
  */
