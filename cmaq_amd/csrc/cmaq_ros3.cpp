@@ -66,6 +66,8 @@ struct cmaq_ros3_ctx {
     size_t tape_bytes = 0;
     int64_t *d_tape_off = nullptr;
     int32_t *d_order = nullptr;       // adjoint queue order, longest cells first
+    const int32_t *user_order = nullptr;   // caller's queue order for the forward (device)
+    int64_t user_order_n = 0;
     int64_t tape_off_cells = 0;
     double replay_ms = 0.0, reverse_ms = 0.0;
     int64_t adj_chunks = 0;
@@ -608,6 +610,7 @@ static int launch_solve(cmaq_ros3_ctx *c, int64_t n, double chemstep, const doub
     HIP_TRY(c, hipEventRecord(c->ev[1], s));
     SolveParams P;
     solve_params(c, P, n, chemstep, rki, y, nacc, nrej);
+    if (c->user_order && c->user_order_n == n) P.order = c->user_order;
     void *args[] = {&P};
     HIP_TRY(c, hipLaunchKernel(c->mk->solve_kernel, dim3(c->grid_blocks), dim3(kSolveBlock), args, 0, s));
     HIP_TRY(c, hipEventRecord(c->ev[2], s));
@@ -764,6 +767,14 @@ extern "C" int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *c, int64_t n, double chem
     return check_fail(c, small, fail);
 }
 
+extern "C" int cmaq_ros3_set_queue_order(cmaq_ros3_ctx *c, const int32_t *order, int64_t n)
+{
+    if (!c || n < 0) return CMAQ_ROS3_E_ARG;
+    c->user_order = order;
+    c->user_order_n = order ? n : 0;
+    return 0;
+}
+
 extern "C" int cmaq_ros3_last_adjoint_timing(const cmaq_ros3_ctx *c, double *replay_ms, double *reverse_ms,
                                              int64_t *chunks)
 {
@@ -899,7 +910,8 @@ extern "C" int cmaq_ros3_run_steps_device(cmaq_ros3_ctx *c, int64_t n, double ch
     // (re)capture when anything the graph bakes in has changed
     SolveParams P;
     solve_params(c, P, n, chemstep, c->d_rki, y, c->d_step_na, c->d_step_nr);
-    std::vector<const void *> key = {(const void *)(intptr_t)n, temp, pres, h2o, rj, het, sea, y, y_ckpt, nacc,
+    if (c->user_order && c->user_order_n == n) P.order = c->user_order;
+    std::vector<const void *> key = {(const void *)P.order, (const void *)(intptr_t)n, temp, pres, h2o, rj, het, sea, y, y_ckpt, nacc,
                                      nrej, c->mk, c->d_ws, (const void *)(intptr_t)c->ws_stride, c->d_rki, c->d_step_na,
                                      (const void *)(intptr_t)P.refill_min, (const void *)(intptr_t)c->grid_blocks};
     double cs = chemstep;

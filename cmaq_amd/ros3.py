@@ -87,6 +87,7 @@ def lib():
         L.cmaq_ros3_run_steps_device.argtypes = [vp, C.c_int64, C.c_double, C.c_int32] + [vp] * 10 + \
             [C.POINTER(_Fail), vp]
         L.cmaq_ros3_last_steps_timing.argtypes = [vp, _f64p]
+        L.cmaq_ros3_set_queue_order.argtypes = [vp, vp, C.c_int64]
         L.cmaq_ros3_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
         L.cmaq_ros3_species_order.argtypes = [vp, _i32p]
         L.cmaq_ros3_last_error.argtypes = [vp]
@@ -279,7 +280,7 @@ class Ros3Solver:
                 continue
             if not (t.is_cuda and t.is_contiguous()):
                 raise ValueError(f"{name} must be a contiguous CUDA tensor")
-            want = torch.int32 if name in ("nacc", "nrej", "nacc_fwd") else torch.float64
+            want = torch.int32 if name in ("nacc", "nrej", "nacc_fwd", "order") else torch.float64
             if t.dtype != want:
                 raise ValueError(f"{name} must be {want}")
             if t.numel() != rows * n:
@@ -304,6 +305,17 @@ class Ros3Solver:
         if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
             raise self._err(rc, fail)
         return fail if rc != 0 else None
+
+    def set_queue_order(self, order):
+        """Hands cells to lanes in this order (int32 CUDA tensor, a permutation
+        of the cells) for forward calls of the same size; None clears it.
+        The tensor must stay alive while set."""
+        if order is not None:
+            self._check([("order", order, 1)], order.numel())
+        self._order = order
+        rc = lib().cmaq_ros3_set_queue_order(self._ctx, _ptr(order), 0 if order is None else order.numel())
+        if rc != 0:
+            raise self._err(rc)
 
     def last_steps_timing(self):
         v = C.c_double()

@@ -188,6 +188,18 @@ int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep
                              const int32_t *n_accepted, double *lambda,
                              double *grad_rki, cmaq_ros3_fail *fail, void *stream);
 
+/*
+ * Scheduling hint for the forward calls (cmaq_ros3_run_device,
+ * cmaq_ros3_solve_device and cmaq_ros3_run): cells are handed to lanes in
+ * the order order[0..n-1] (a device array holding a permutation of
+ * 0..n-1, e.g. the cells sorted by their attempted-step counts of the
+ * previous sync step, longest first) whenever a call has exactly n cells.
+ * Results do not depend on the order.  NULL clears the hint.  The array must
+ * stay valid while the hint is set.  No counterpart in the reference (whose
+ * LORDERING pass, rbdriver.F:430-434, 682-724, reorders cells into blocks).
+ */
+int cmaq_ros3_set_queue_order(cmaq_ros3_ctx *ctx, const int32_t *order, int64_t n);
+
 /* Device time of the last cmaq_ros3_adjoint_device call: replay (tape
  * writing) and reverse kernels, and the number of cell chunks. */
 int cmaq_ros3_last_adjoint_timing(const cmaq_ros3_ctx *ctx, double *replay_ms,
