@@ -124,6 +124,26 @@ def measured_traffic_per_step(mech_name, variant):
     return None, None
 
 
+def row_stream_ceiling():
+    """Read throughput of the solver's access stream (scattered 512-byte lane
+    rows from per-wave 2.75 MB regions) measured by cmaq_amd/hbm_rows at 8
+    resident waves per CU, the solver's occupancy; None if the probe is absent."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cmaq_amd", "hbm_rows")
+    if not os.path.exists(exe):
+        return None
+    try:
+        r = subprocess.run([exe, "4000"], capture_output=True, text=True, timeout=120)
+    except subprocess.TimeoutExpired:
+        return None
+    best = None
+    for line in r.stdout.splitlines():
+        f = line.split()
+        if len(f) == 8 and f[1] == "8" and f[3] == "8":
+            best = float(f[7])
+    return best
+
+
 def loop_leg(args, mech, solver, dev):
     """12 sync steps of the mixed workload through the captured graph."""
     import torch
@@ -335,6 +355,11 @@ def main():
         ratio, info = adjoint_leg(args, mech, solver, dev)
         out["fwd_adj_ratio"] = ratio
         out["adjoint"] = info
+    ceiling = row_stream_ceiling()
+    if ceiling:
+        out["roofline"]["row_stream_ceiling"] = ceiling
+        if traffic:
+            out["roofline"]["traffic_frac_of_row_stream_ceiling"] = traffic / solve_s / 1e9 / ceiling
     if not args.no_cpu:
         print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args, mech, ncells, nlay_cells)

@@ -17,6 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 GEN = os.path.join(CSRC, "gen")
 OBJ = os.path.join(GEN, "obj")
 LIB = os.path.join(HERE, "libcmaq_ros3.so")
+PROBE = os.path.join(HERE, "hbm_rows")       # memory-stream throughput probe (bench.py)
 MECHS = {"cb6r5_ae7_aq": "cb6r5_ae7_aq.json", "cracmm2": "cracmm2.json"}
 # code-generation variants per mechanism: (register budget, load lookahead,
 # LU schedule of programs.jac_lu_ops); the first is the default unless
@@ -126,6 +127,11 @@ def build(names=None, verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(LIB + ".tmp", LIB)
+    cmd = [HIPCC] + HIP_FLAGS + ["-o", PROBE + ".tmp", os.path.join(CSRC, "hbm_rows.hip")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"probe build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(PROBE + ".tmp", PROBE)
     if verbose:
         print(f"built {LIB}")
     return LIB
