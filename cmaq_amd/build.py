@@ -24,13 +24,16 @@ MECHS = {"cb6r5_ae7_aq": "cb6r5_ae7_aq.json", "cracmm2": "cracmm2.json"}
 # tuning.json names another (written by cmaq_amd/autotune.py from timings on
 # the GPU)
 VARIANTS = {
-    "cb6r5_ae7_aq": [(110, 32, "column"), (110, 32, "tile10x10")],
+    "cb6r5_ae7_aq": [(110, 32, "column"), (110, 32, "column+k1"), (110, 32, "tile10x10")],
     "cracmm2": [(110, 32, "column"), (110, 32, "tile12x12"), (110, 8, "tile12x12")],
 }
 
 
 def variant_name(b, l, order):
-    suffix = "" if order == "column" else "_t" + order[4:]
+    base, _, extra = order.partition("+")
+    suffix = "" if base == "column" else "_t" + base[4:]
+    if extra:
+        suffix += "_" + extra
     return f"b{b}_l{l}{suffix}"
 
 TUNING = os.path.join(HERE, "tuning.json")

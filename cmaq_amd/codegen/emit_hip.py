@@ -141,14 +141,22 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     body += _function("feval", ", int ys_off_", items, off, budget, {"YS": "ys_off_"})
     b("")
 
-    ops = jac_lu_ops(sym, jac_order)
-    items, cnt = allocate(ops, {"LU", "VD"}, budget, lookahead, entry_all)
+    # "<order>+k1": stage 1's forward substitution fused into the factorization
+    base_order, fuse_k1 = jac_order.split("+")[0], jac_order.endswith("+k1")
+    ops = jac_lu_ops(sym, base_order, fuse_k1)
+    items, cnt = allocate(ops, {"LU", "VD", "K1"} if fuse_k1 else {"LU", "VD"}, budget, lookahead, entry_all)
     stats["jac_lu"] = _stat(ops, cnt)
     body += _function("jac_lu", ", double gdtinv", items, off, budget)
     b("")
+    if fuse_k1:
+        # the adjoint's refactorization has no f(Y) to substitute
+        ops = jac_lu_ops(sym, base_order)
+        items, cnt = allocate(ops, {"LU", "VD"}, budget, lookahead, entry_all)
+        body += _function("jac_lu_plain", ", double gdtinv", items, off, budget)
+        b("")
 
     for stage in (1, 2, 3):
-        ops = solve_ops(sym, stage)
+        ops = solve_ops(sym, stage, backward_only=fuse_k1 and stage == 1)
         outs = {f"K{stage}"}
         if stage == 1:
             # YP = Y + K1 (rbsolver.F:395-398) as soon as K1(i) is final
@@ -258,11 +266,13 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w("}")
     w(f"const int32_t INEW2OLD_HOST[NS] = {{{', '.join(str(v) for v in sym.inew2old)}}};")
     w("")
+    w(f"#define ROS3_JAC_LU_PLAIN {'jac_lu_plain' if fuse_k1 else 'jac_lu'}")
     w("#include \"../ros3_solver_kernel.inc\"")
     w("#include \"../ros3_adjoint_kernel.inc\"")
     w("#undef LD")
     w("#undef ST")
     w("#undef ROS3_PROLOGUE")
+    w("#undef ROS3_JAC_LU_PLAIN")
     w(f"}} }} // namespace ros3::{cname}")
     w("")
     w(f"extern \"C\" const ros3::MechKernels ros3_kernels_{cname} = {{")

@@ -124,7 +124,50 @@ def _entry_ops(sym, terms, s, diag):
     return ops
 
 
-def jac_lu_ops(sym, order="column"):
+def _fuse_k1_forward(sym, ops):
+    """Appends stage 1's forward substitution (rbsolve.F:78-128 with
+    K1 = -F, rbsolver.F:386-388) to a column-ordered factorization: once
+    column j's L entries are scaled, K1(i) -= L(i,j) * K1(j) for its rows i.
+    Each K1(i) still starts from -F(i) and takes its terms in ascending j, the
+    reference's order, while the L entries are still in registers."""
+    day = sym.day
+    first = {}
+    for j in range(sym.ns):
+        first.setdefault(j, j)
+        for s in day.l_scale[j]:
+            first.setdefault(day.rows[s], j)
+            first[day.rows[s]] = min(first[day.rows[s]], j)
+    init_at = {}
+    for i, j in first.items():
+        init_at.setdefault(j, []).append(i)
+    out = []
+    for op in ops:
+        if op.fmt == "{0} = 1.0 / {1};" and op.args[0][0] == "VD":
+            j = op.args[0][1]
+            for i in sorted(init_at.get(j, [])):
+                out.append(Op("{0} = -{1};", [("K1", i), ("F", i)], False))
+        out.append(op)
+        # after the last L scaling of column j, apply column j of L to K1
+        if op.fmt == "{0} = {0} * {1};" and op.args[1][0] == "VD":
+            j = op.args[1][1]
+            if op.args[0][1] == day.l_scale[j][-1]:
+                for s in sorted(day.l_scale[j], key=lambda s: day.rows[s]):
+                    out.append(Op("{0} = {0} - {1} * {2};", [("K1", day.rows[s]), ("LU", s), ("K1", j)], True))
+    return out
+
+
+def jac_lu_ops(sym, order="column", fuse_k1=False):
+    """Jacobian, shift and LU in the given schedule; with fuse_k1 (column
+    order only) also stage 1's forward substitution."""
+    ops = _jac_lu_ops(sym, order)
+    if fuse_k1:
+        if order != "column":
+            raise ValueError("the K1 forward substitution is fused into the column schedule only")
+        ops = _fuse_k1_forward(sym, ops)
+    return ops
+
+
+def _jac_lu_ops(sym, order="column"):
     day = sym.day
     n = sym.ns
     terms = _jac_terms(sym)
@@ -252,12 +295,13 @@ def jac_lu_ops(sym, order="column"):
     return ops
 
 
-def solve_ops(sym, stage):
-    """Stage solve: x = K<stage>, right-hand side per rbsolver.F:386-422."""
+def solve_ops(sym, stage, backward_only=False):
+    """Stage solve: x = K<stage>, right-hand side per rbsolver.F:386-422.
+    backward_only: the forward substitution was fused into jac_lu."""
     day = sym.day
     x = f"K{stage}"
     ops = []
-    for i in range(sym.ns):
+    for i in ([] if backward_only else range(sym.ns)):
         if stage == 1:
             ops.append(Op("{0} = -{1};", [(x, i), ("F", i)], False))
         elif stage == 2:

@@ -38,12 +38,12 @@ class Programs:
     """The forward and adjoint programs of one mechanism as Python functions
     on a workspace dict of lists (species in the solver's order)."""
 
-    def __init__(self, sym, jac_order="column"):
+    def __init__(self, sym, jac_order="column", fuse_k1=False):
         self.sym = sym
         fe = P.feval_ops(sym, "scatter")
         self.feval = compile_ops(fe)
-        self.jac_lu = compile_ops(P.jac_lu_ops(sym, jac_order), ["gdtinv"])
-        self.solve = [compile_ops(P.solve_ops(sym, s), ["x1", "x2"]) for s in (1, 2, 3)]
+        self.jac_lu = compile_ops(P.jac_lu_ops(sym, jac_order, fuse_k1), ["gdtinv"])
+        self.solve = [compile_ops(P.solve_ops(sym, s, fuse_k1 and s == 1), ["x1", "x2"]) for s in (1, 2, 3)]
         self.yp = compile_ops(P.yp_ops(sym))
         self.seed = compile_ops(P.adj_seed_ops(sym))
         self.ts = {x: compile_ops(P.tsolve_ops(sym, x)) for x in ("U1", "U2", "U3")}

@@ -20,11 +20,13 @@ def _cell(mech, o, seed):
     return rki[:, 0].copy(), d["y"][:, 0].copy()
 
 
-@pytest.mark.parametrize("jac_order", ["column", "row", "group3", "tile10x10", "tile6x10"])
-def test_forward_programs_bit_exact(cb6r5, oracle_cb6r5, jac_order):
+@pytest.mark.parametrize("jac_order,fuse", [("column", False), ("row", False), ("group3", False),
+                                            ("tile10x10", False), ("tile6x10", False), ("column", True)])
+def test_forward_programs_bit_exact(cb6r5, oracle_cb6r5, jac_order, fuse):
     """Every LU schedule keeps each entry's reference operation sequence, so
-    all reproduce the oracle bit for bit."""
-    o, pg = oracle_cb6r5, Programs(analyse(cb6r5), jac_order)
+    all reproduce the oracle bit for bit; so does stage 1's forward
+    substitution fused into the column factorization."""
+    o, pg = oracle_cb6r5, Programs(analyse(cb6r5), jac_order, fuse)
     inew = o.inew2old() - 1
     for seed in (1, 4):
         rki, y0 = _cell(cb6r5, o, seed)
