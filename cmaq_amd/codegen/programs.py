@@ -125,46 +125,42 @@ def _entry_ops(sym, terms, s, diag):
 
 
 def _fuse_k1_forward(sym, ops):
-    """Appends stage 1's forward substitution (rbsolve.F:78-128 with
-    K1 = -F, rbsolver.F:386-388) to a column-ordered factorization: once
-    column j's L entries are scaled, K1(i) -= L(i,j) * K1(j) for its rows i.
-    Each K1(i) still starts from -F(i) and takes its terms in ascending j, the
-    reference's order, while the L entries are still in registers."""
+    """Interleaves stage 1's forward substitution (rbsolve.F:78-128 with
+    K1 = -F, rbsolver.F:386-388) with the factorization: right after L(i,j)
+    is scaled, K1(i) -= L(i,j) * K1(j), while L(i,j) is still in a register.
+    Every schedule of jac_lu_ops scales a row's L entries in ascending column
+    order and only after all of L(j,:) is scaled, so each K1(i) starts from
+    -F(i) and takes its terms in ascending j -- the reference's order -- and
+    K1(j) is final when it is used."""
     day = sym.day
-    first = {}
-    for j in range(sym.ns):
-        first.setdefault(j, j)
-        for s in day.l_scale[j]:
-            first.setdefault(day.rows[s], j)
-            first[day.rows[s]] = min(first[day.rows[s]], j)
-    init_at = {}
-    for i, j in first.items():
-        init_at.setdefault(j, []).append(i)
+    started = set()
     out = []
+
+    def start(i):
+        if i not in started:
+            started.add(i)
+            out.append(Op("{0} = -{1};", [("K1", i), ("F", i)], False))
+
     for op in ops:
-        if op.fmt == "{0} = 1.0 / {1};" and op.args[0][0] == "VD":
-            j = op.args[0][1]
-            for i in sorted(init_at.get(j, [])):
-                out.append(Op("{0} = -{1};", [("K1", i), ("F", i)], False))
         out.append(op)
-        # after the last L scaling of column j, apply column j of L to K1
-        if op.fmt == "{0} = {0} * {1};" and op.args[1][0] == "VD":
-            j = op.args[1][1]
-            if op.args[0][1] == day.l_scale[j][-1]:
-                for s in sorted(day.l_scale[j], key=lambda s: day.rows[s]):
-                    out.append(Op("{0} = {0} - {1} * {2};", [("K1", day.rows[s]), ("LU", s), ("K1", j)], True))
+        if op.fmt == "{0} = {0} * {1};" and op.args[1][0] == "VD" and op.args[0][0] == "LU":
+            s_, j = op.args[0][1], op.args[1][1]
+            i = day.rows[s_]
+            if i > j:
+                start(j)
+                start(i)
+                out.append(Op("{0} = {0} - {1} * {2};", [("K1", i), ("LU", s_), ("K1", j)], True))
+    for i in range(sym.ns):      # rows without L entries
+        if i not in started:
+            start(i)
     return out
 
 
 def jac_lu_ops(sym, order="column", fuse_k1=False):
-    """Jacobian, shift and LU in the given schedule; with fuse_k1 (column
-    order only) also stage 1's forward substitution."""
+    """Jacobian, shift and LU in the given schedule; with fuse_k1 also stage
+    1's forward substitution."""
     ops = _jac_lu_ops(sym, order)
-    if fuse_k1:
-        if order != "column":
-            raise ValueError("the K1 forward substitution is fused into the column schedule only")
-        ops = _fuse_k1_forward(sym, ops)
-    return ops
+    return _fuse_k1_forward(sym, ops) if fuse_k1 else ops
 
 
 def _jac_lu_ops(sym, order="column"):
