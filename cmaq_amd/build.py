@@ -24,7 +24,7 @@ MECHS = {"cb6r5_ae7_aq": "cb6r5_ae7_aq.json", "cracmm2": "cracmm2.json"}
 # tuning.json names another (written by cmaq_amd/autotune.py from timings on
 # the GPU)
 VARIANTS = {
-    "cb6r5_ae7_aq": [(110, 32, "column"), (110, 32, "column+k1"), (110, 32, "tile10x10")],
+    "cb6r5_ae7_aq": [(110, 32, "column+k1"), (110, 32, "column+step"), (110, 32, "column")],
     "cracmm2": [(110, 8, "tile12x12"), (110, 8, "tile12x12+k1"), (110, 32, "column")],
 }
 

@@ -23,6 +23,7 @@ zeros (their only contributions are photolysis terms with J = 0), so every
 night result is bit-identical to the reference's night pattern.
 """
 
+import re
 import struct
 
 from .programs import (adj_a_ops, adj_b_ops, adj_seed_ops, adj_u1_ops, adj_u2_ops, feval_ops,
@@ -95,6 +96,18 @@ def _function(name, params, items, off, budget, dynamic=None):
     return w
 
 
+def _remap(ops, arrays=None, scalars=None):
+    """The same operations on renamed arrays / scalar parameters."""
+    arrays, scalars = arrays or {}, scalars or {}
+    pat = re.compile(r"\b(" + "|".join(map(re.escape, scalars)) + r")\b") if scalars else None
+    out = []
+    for op in ops:
+        fmt = pat.sub(lambda m: scalars[m.group(1)], op.fmt) if pat else op.fmt
+        args = [(arrays.get(a[0], a[0]), a[1]) if isinstance(a, tuple) else scalars.get(a, a) for a in op.args]
+        out.append(Op(fmt, args, op.rmw))
+    return out
+
+
 def _stat(ops, counts):
     return (len(ops), counts.get("load", 0), counts.get("store", 0))
 
@@ -134,44 +147,58 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     body = []
     b = body.append
 
-    # f(y): y at a runtime workspace offset (OFF_Y or OFF_YP)
-    ops = feval_ops(sym, FEVAL_ORDER)
-    items, cnt = allocate(ops, {"F"}, budget, lookahead, entry_all)
-    stats["feval"] = _stat(ops, cnt)
-    body += _function("feval", ", int ys_off_", items, off, budget, {"YS": "ys_off_"})
-    b("")
+    # "<order>+k1": stage 1's forward substitution fused into the factorization;
+    # "+step": the whole attempted step (f(Y), Jacobian + LU, three stage
+    # solves, f(YP)) as one function, so registers carry values across the
+    # phases and the callee-saved registers are saved once per step
+    base_order = jac_order.split("+")[0]
+    merge = "+step" in jac_order
+    fuse_k1 = jac_order.endswith("+k1") or merge
 
-    # "<order>+k1": stage 1's forward substitution fused into the factorization
-    base_order, fuse_k1 = jac_order.split("+")[0], jac_order.endswith("+k1")
-    ops = jac_lu_ops(sym, base_order, fuse_k1)
-    items, cnt = allocate(ops, {"LU", "VD", "K1"} if fuse_k1 else {"LU", "VD"}, budget, lookahead, entry_all)
-    stats["jac_lu"] = _stat(ops, cnt)
-    body += _function("jac_lu", ", double gdtinv", items, off, budget)
-    b("")
+    def stage1_backward():
+        # YP = Y + K1 (rbsolver.F:395-398) as soon as K1(i) is final
+        out = []
+        for op in solve_ops(sym, 1, backward_only=fuse_k1):
+            out.append(op)
+            if op.fmt == "{0} = {0} * {1};" and op.args[0][0] == "K1":
+                i = op.args[0][1]
+                out.append(Op("{0} = {1} + {2};", [("YP", i), ("Y", i), ("K1", i)], False))
+        return out
+
+    if merge:
+        step = (_remap(feval_ops(sym, FEVAL_ORDER), {"YS": "Y"}) + jac_lu_ops(sym, base_order, True) +
+                stage1_backward() + _remap(feval_ops(sym, FEVAL_ORDER), {"YS": "YP"}) +
+                _remap(solve_ops(sym, 2), scalars={"x1": "x21"}) +
+                _remap(solve_ops(sym, 3), scalars={"x1": "x31", "x2": "x32"}))
+        items, cnt = allocate(step, {"K1", "K2", "K3"}, budget, lookahead, entry_all)
+        stats["ros3_step"] = _stat(step, cnt)
+        body += _function("ros3_step", ", double gdtinv, double x21, double x31, double x32", items, off, budget)
+        b("")
+    else:
+        # f(y): y at a runtime workspace offset (OFF_Y or OFF_YP)
+        ops = feval_ops(sym, FEVAL_ORDER)
+        items, cnt = allocate(ops, {"F"}, budget, lookahead, entry_all)
+        stats["feval"] = _stat(ops, cnt)
+        body += _function("feval", ", int ys_off_", items, off, budget, {"YS": "ys_off_"})
+        b("")
+        ops = jac_lu_ops(sym, base_order, fuse_k1)
+        items, cnt = allocate(ops, {"LU", "VD", "K1"} if fuse_k1 else {"LU", "VD"}, budget, lookahead, entry_all)
+        stats["jac_lu"] = _stat(ops, cnt)
+        body += _function("jac_lu", ", double gdtinv", items, off, budget)
+        b("")
+        for stage in (1, 2, 3):
+            ops = stage1_backward() if stage == 1 else solve_ops(sym, stage)
+            outs = {"K1", "YP"} if stage == 1 else {f"K{stage}"}
+            items, cnt = allocate(ops, outs, budget, lookahead, entry_all)
+            stats[f"solve{stage}"] = _stat(ops, cnt)
+            params = "" if stage == 1 else (", double x1" if stage == 2 else ", double x1, double x2")
+            body += _function(f"solve{stage}", params, items, off, budget)
+            b("")
     if fuse_k1:
         # the adjoint's refactorization has no f(Y) to substitute
         ops = jac_lu_ops(sym, base_order)
         items, cnt = allocate(ops, {"LU", "VD"}, budget, lookahead, entry_all)
         body += _function("jac_lu_plain", ", double gdtinv", items, off, budget)
-        b("")
-
-    for stage in (1, 2, 3):
-        ops = solve_ops(sym, stage, backward_only=fuse_k1 and stage == 1)
-        outs = {f"K{stage}"}
-        if stage == 1:
-            # YP = Y + K1 (rbsolver.F:395-398) as soon as K1(i) is final
-            fused = []
-            for op in ops:
-                fused.append(op)
-                if op.fmt == "{0} = {0} * {1};" and op.args[0][0] == "K1":
-                    i = op.args[0][1]
-                    fused.append(Op("{0} = {1} + {2};", [("YP", i), ("Y", i), ("K1", i)], False))
-            ops = fused
-            outs.add("YP")
-        items, cnt = allocate(ops, outs, budget, lookahead, entry_all)
-        stats[f"solve{stage}"] = _stat(ops, cnt)
-        params = "" if stage == 1 else (", double x1" if stage == 2 else ", double x1, double x2")
-        body += _function(f"solve{stage}", params, items, off, budget)
         b("")
 
     # discrete adjoint of one accepted step
@@ -267,12 +294,15 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w(f"const int32_t INEW2OLD_HOST[NS] = {{{', '.join(str(v) for v in sym.inew2old)}}};")
     w("")
     w(f"#define ROS3_JAC_LU_PLAIN {'jac_lu_plain' if fuse_k1 else 'jac_lu'}")
+    if merge:
+        w("#define ROS3_MERGED_STEP 1")
     w("#include \"../ros3_solver_kernel.inc\"")
     w("#include \"../ros3_adjoint_kernel.inc\"")
     w("#undef LD")
     w("#undef ST")
     w("#undef ROS3_PROLOGUE")
     w("#undef ROS3_JAC_LU_PLAIN")
+    w("#undef ROS3_MERGED_STEP")
     w(f"}} }} // namespace ros3::{cname}")
     w("")
     w(f"extern \"C\" const ros3::MechKernels ros3_kernels_{cname} = {{")
