@@ -20,20 +20,23 @@ LIB = os.path.join(HERE, "libcmaq_ros3.so")
 PROBE = os.path.join(HERE, "hbm_rows")       # memory-stream throughput probe (bench.py)
 MECHS = {"cb6r5_ae7_aq": "cb6r5_ae7_aq.json", "cracmm2": "cracmm2.json"}
 # code-generation variants per mechanism: (register budget, load lookahead,
-# LU schedule of programs.jac_lu_ops); the first is the default unless
-# tuning.json names another (written by cmaq_amd/autotune.py from timings on
-# the GPU)
+# LU schedule of programs.jac_lu_ops with options "+k1" (stage-1 forward
+# substitution fused into the LU), "+step" (one function per attempted step)
+# and "+ldsN" (N doubles per lane of LDS as a second register tier)); the first
+# is the default unless tuning.json names another (written by
+# cmaq_amd/autotune.py from timings on the GPU).  The table keeps the winners;
+# tuning.json records the others that were measured.
 VARIANTS = {
-    "cb6r5_ae7_aq": [(110, 32, "column+k1"), (110, 32, "column+step"), (110, 32, "column")],
+    "cb6r5_ae7_aq": [(110, 32, "column+k1"), (110, 32, "column")],
     "cracmm2": [(110, 8, "tile12x12"), (110, 8, "tile12x12+k1"), (110, 32, "column")],
 }
 
 
 def variant_name(b, l, order):
-    base, _, extra = order.partition("+")
+    base, *extra = order.split("+")
     suffix = "" if base == "column" else "_t" + base[4:]
-    if extra:
-        suffix += "_" + extra
+    for e in extra:
+        suffix += "_" + e
     return f"b{b}_l{l}{suffix}"
 
 TUNING = os.path.join(HERE, "tuning.json")

@@ -85,11 +85,18 @@ def _function(name, params, items, off, budget, dynamic=None):
             return f"({arr}_off + {i})"
         return str(off[arr] + i)
 
+    if any(it[0] in ("lds_load", "lds_store") for it in items):
+        # this wave's slice of the LDS tier: entry q of the lane at lt[q * 64]
+        w.append("    double *const lt = lds_tier + (int)(threadIdx.x >> 6) * (LDS_SLOTS * kWave) + __lane_id();")
     for it in items:
         if it[0] == "load":
             w.append(f"    r{it[1]} = LD({addr(it[2])});")
         elif it[0] == "store":
             w.append(f"    ST({addr(it[2])}, r{it[1]});")
+        elif it[0] == "lds_load":
+            w.append(f"    r{it[1]} = lt[{it[2]} * kWave];")
+        elif it[0] == "lds_store":
+            w.append(f"    lt[{it[2]} * kWave] = r{it[1]};")
         else:
             w.append("    " + it[1])
     w.append("}")
@@ -109,7 +116,8 @@ def _remap(ops, arrays=None, scalars=None):
 
 
 def _stat(ops, counts):
-    return (len(ops), counts.get("load", 0), counts.get("store", 0))
+    return (len(ops), counts.get("load", 0), counts.get("store", 0)) + \
+        ((counts.get("lds_load", 0), counts.get("lds_store", 0)) if "lds_load" in counts else ())
 
 
 def waves_per_eu(budget):
@@ -152,8 +160,12 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     # solves, f(YP)) as one function, so registers carry values across the
     # phases and the callee-saved registers are saved once per step
     base_order = jac_order.split("+")[0]
-    merge = "+step" in jac_order
-    fuse_k1 = jac_order.endswith("+k1") or merge
+    flags = jac_order.split("+")[1:]
+    merge = "step" in flags
+    fuse_k1 = "k1" in flags or merge
+    # "+ldsN": N doubles per lane of LDS as a second tier for the forward
+    # functions (clean copies of evicted values that are read again)
+    lds = next((int(f[3:]) for f in flags if f.startswith("lds")), 0)
 
     def stage1_backward():
         # YP = Y + K1 (rbsolver.F:395-398) as soon as K1(i) is final
@@ -170,26 +182,26 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
                 stage1_backward() + _remap(feval_ops(sym, FEVAL_ORDER), {"YS": "YP"}) +
                 _remap(solve_ops(sym, 2), scalars={"x1": "x21"}) +
                 _remap(solve_ops(sym, 3), scalars={"x1": "x31", "x2": "x32"}))
-        items, cnt = allocate(step, {"K1", "K2", "K3"}, budget, lookahead, entry_all)
+        items, cnt = allocate(step, {"K1", "K2", "K3"}, budget, lookahead, entry_all, lds)
         stats["ros3_step"] = _stat(step, cnt)
         body += _function("ros3_step", ", double gdtinv, double x21, double x31, double x32", items, off, budget)
         b("")
     else:
         # f(y): y at a runtime workspace offset (OFF_Y or OFF_YP)
         ops = feval_ops(sym, FEVAL_ORDER)
-        items, cnt = allocate(ops, {"F"}, budget, lookahead, entry_all)
+        items, cnt = allocate(ops, {"F"}, budget, lookahead, entry_all, lds)
         stats["feval"] = _stat(ops, cnt)
         body += _function("feval", ", int ys_off_", items, off, budget, {"YS": "ys_off_"})
         b("")
         ops = jac_lu_ops(sym, base_order, fuse_k1)
-        items, cnt = allocate(ops, {"LU", "VD", "K1"} if fuse_k1 else {"LU", "VD"}, budget, lookahead, entry_all)
+        items, cnt = allocate(ops, {"LU", "VD", "K1"} if fuse_k1 else {"LU", "VD"}, budget, lookahead, entry_all, lds)
         stats["jac_lu"] = _stat(ops, cnt)
         body += _function("jac_lu", ", double gdtinv", items, off, budget)
         b("")
         for stage in (1, 2, 3):
             ops = stage1_backward() if stage == 1 else solve_ops(sym, stage)
             outs = {"K1", "YP"} if stage == 1 else {f"K{stage}"}
-            items, cnt = allocate(ops, outs, budget, lookahead, entry_all)
+            items, cnt = allocate(ops, outs, budget, lookahead, entry_all, lds)
             stats[f"solve{stage}"] = _stat(ops, cnt)
             params = "" if stage == 1 else (", double x1" if stage == 2 else ", double x1, double x2")
             body += _function(f"solve{stage}", params, items, off, budget)
@@ -239,6 +251,9 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w("    const int vo = __lane_id() * 8;")
     w("#define LD(k) __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, (k) * (kWave * 8), 0))")
     w("#define ST(k, v) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (double)(v)), rs, vo, (k) * (kWave * 8), 0)")
+    if lds:
+        w(f"constexpr int LDS_SLOTS = {lds};")
+        w("__shared__ double lds_tier[(256 / kWave) * LDS_SLOTS * kWave];   // 4 waves per workgroup")
     w("")
     out += body
     # cell load / store: inputs and results are cell-fastest global arrays
