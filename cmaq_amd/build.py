@@ -18,17 +18,27 @@ GEN = os.path.join(CSRC, "gen")
 OBJ = os.path.join(GEN, "obj")
 LIB = os.path.join(HERE, "libcmaq_ros3.so")
 PROBE = os.path.join(HERE, "hbm_rows")       # memory-stream throughput probe (bench.py)
-MECHS = {"cb6r5_ae7_aq": "cb6r5_ae7_aq.json", "cracmm2": "cracmm2.json"}
+MECHS = {n: n + ".json" for n in (
+    "cb6r5_ae7_aq", "cracmm2", "cb6r3_ae7_aq", "cb6r5m_ae7_aq", "cracmm1_aq", "cracmm1amore_aq",
+    "racm2_ae6_aq", "saprc07tc_ae6_aq", "saprc07tic_ae7i_aq")}
 # code-generation variants per mechanism: (register budget, load lookahead,
 # LU schedule of programs.jac_lu_ops with options "+k1" (stage-1 forward
-# substitution fused into the LU), "+step" (one function per attempted step)
-# and "+ldsN" (N doubles per lane of LDS as a second register tier)); the first
-# is the default unless tuning.json names another (written by
-# cmaq_amd/autotune.py from timings on the GPU).  The table keeps the winners;
-# tuning.json records the others that were measured.
+# substitution fused into the LU), "+step" (one function per attempted step),
+# "+ldsN" (N doubles per lane of LDS as a second register tier) and "+sbN" (a
+# scheduling barrier every N statements)); the first is the default unless
+# tuning.json names another (written by cmaq_amd/autotune.py from timings on
+# the GPU).  The table keeps the winners; tuning.json records the others that
+# were measured.
 VARIANTS = {
     "cb6r5_ae7_aq": [(110, 32, "column+k1"), (110, 32, "column")],
-    "cracmm2": [(110, 8, "tile12x12"), (110, 8, "tile12x12+k1"), (110, 32, "column")],
+    "cracmm2": [(110, 8, "tile12x12+k1")],
+    "cb6r3_ae7_aq": [(110, 32, "column+k1")],
+    "cb6r5m_ae7_aq": [(110, 32, "column+k1")],
+    "racm2_ae6_aq": [(110, 8, "tile12x12+k1")],
+    "cracmm1_aq": [(110, 8, "tile12x12+k1")],
+    "cracmm1amore_aq": [(110, 8, "tile12x12+k1")],
+    "saprc07tc_ae6_aq": [(110, 8, "tile12x12+k1")],
+    "saprc07tic_ae7i_aq": [(110, 8, "tile12x12+k1")],
 }
 
 
@@ -48,6 +58,12 @@ ARCH = "gfx950"
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
              "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
+# generated kernels: skip the machine scheduler's two re-scheduling stages
+# (high register pressure, low occupancy).  They take half the compile time of
+# the long straight-line functions (SAPRC07TIC: 637 s -> 227 s) and the
+# kernels run as fast without them (CB6r5: 36.6 vs 36.5 M cell-steps/s).
+KERNEL_FLAGS = ["-mllvm", "-amdgpu-disable-unclustered-high-rp-reschedule",
+                "-mllvm", "-amdgpu-disable-clustered-low-occupancy-reschedule"]
 
 
 def load_mech(name):
@@ -87,8 +103,16 @@ def generate(names=None):
                            refill_min=refill, jac_order=order)
             path = os.path.join(GEN, f"ros3_{cname}.hip")
             _write_if_changed(path, text)
-            srcs.append(path)
+            srcs.append((path, KERNEL_FLAGS))
             cnames.append(cname)
+    if set(names) == set(MECHS):
+        # drop sources and objects of variants no longer in the table
+        keep = {f"ros3_{c}" for c in cnames}
+        for d, ext in ((GEN, ".hip"), (OBJ, ".o")):
+            for f in os.listdir(d) if os.path.isdir(d) else []:
+                stem = f[:f.index(".")] if "." in f else f
+                if f.startswith("ros3_") and f.endswith((ext, ext + ".cmd")) and stem not in keep:
+                    os.remove(os.path.join(d, f))
     reg = ["// Generated by cmaq_amd/build.py; do not edit.", "#include \"../ros3_registry.h\""]
     for name in cnames:
         reg.append(f"extern \"C\" const ros3::MechKernels ros3_kernels_{name};")
@@ -98,33 +122,41 @@ def generate(names=None):
     reg.append(f"extern \"C\" const int ros3_mech_registry_size = {len(cnames)};")
     path = os.path.join(GEN, "registry.cpp")
     _write_if_changed(path, "\n".join(reg) + "\n")
-    srcs.append(path)
+    srcs.append((path, []))
     return srcs
 
 
-def _compile(src, obj, host_only=False):
-    deps = [src, os.path.join(CSRC, "ros3_device.h"), os.path.join(CSRC, "ros3_registry.h"), os.path.join(CSRC, "ros3_solver_kernel.inc"), os.path.join(CSRC, "ros3_adjoint_kernel.inc"),
-            os.path.join(CSRC, "ros3_registry.h"), os.path.join(HERE, "..", "include", "cmaq_ros3.h")]
-    if os.path.exists(obj) and all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps if os.path.exists(d)):
-        return obj
+def _compile(src, obj, host_only=False, extra=(), force=False):
     if host_only:
         cmd = ["g++", "-O2", "-fPIC", "-std=c++17", "-c", src, "-o", obj]
     else:
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj]
+        cmd = [HIPCC] + HIP_FLAGS + list(extra) + ["-c", src, "-o", obj]
+    # an object is reused only if it is newer than its sources and was built
+    # with the same command
+    stamp = obj + ".cmd"
+    same_cmd = os.path.exists(stamp) and open(stamp).read() == " ".join(cmd)
+    deps = [src, os.path.join(CSRC, "ros3_device.h"), os.path.join(CSRC, "ros3_registry.h"), os.path.join(CSRC, "ros3_solver_kernel.inc"), os.path.join(CSRC, "ros3_adjoint_kernel.inc"),
+            os.path.join(CSRC, "ros3_registry.h"), os.path.join(HERE, "..", "include", "cmaq_ros3.h")]
+    if not force and same_cmd and os.path.exists(obj) and \
+            all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps if os.path.exists(d)):
+        return obj
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    with open(stamp, "w") as f:
+        f.write(" ".join(cmd))
     return obj
 
 
-def build(names=None, verbose=True):
+def build(names=None, verbose=True, force=False):
+    """Generate and compile; force=True recompiles every object."""
     os.makedirs(OBJ, exist_ok=True)
     gen_srcs = generate(names)
     jobs = []
-    for src in gen_srcs:
+    for src, extra in gen_srcs:
         base = os.path.splitext(os.path.basename(src))[0]
-        jobs.append((src, os.path.join(OBJ, base + ".o"), src.endswith("registry.cpp")))
-    jobs.append((os.path.join(CSRC, "cmaq_ros3.cpp"), os.path.join(OBJ, "cmaq_ros3.o"), False))
+        jobs.append((src, os.path.join(OBJ, base + ".o"), src.endswith("registry.cpp"), extra, force))
+    jobs.append((os.path.join(CSRC, "cmaq_ros3.cpp"), os.path.join(OBJ, "cmaq_ros3.o"), False, (), force))
     workers = int(os.environ.get("MAX_JOBS", "8"))
     with cf.ThreadPoolExecutor(max_workers=max(1, min(workers, len(jobs)))) as ex:
         objs = list(ex.map(lambda j: _compile(*j), jobs))
@@ -144,4 +176,5 @@ def build(names=None, verbose=True):
 
 
 if __name__ == "__main__":
-    build(sys.argv[1:] or None)
+    args = [a for a in sys.argv[1:] if a != "--force"]
+    build(args or None, force="--force" in sys.argv[1:])

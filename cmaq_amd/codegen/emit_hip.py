@@ -23,6 +23,7 @@ zeros (their only contributions are photolysis terms with J = 0), so every
 night result is bit-identical to the reference's night pattern.
 """
 
+import functools
 import re
 import struct
 
@@ -70,8 +71,13 @@ def structure_hash(m):
     return h
 
 
-def _function(name, params, items, off, budget, dynamic=None):
-    """HIP source of one generated function from allocator items."""
+def _function(name, params, items, off, budget, dynamic=None, sched=0):
+    """HIP source of one generated function from allocator items.
+
+    sched > 0 closes the compiler's instruction-scheduling region after every
+    `sched` statements: the allocator has already placed each load `lookahead`
+    operations ahead of its use, and the scheduler's cost grows faster than
+    linearly with the region length (one region per function otherwise)."""
     dynamic = dynamic or {}
     w = [f"__device__ __noinline__ void {name}(double *base_{params})", "{",
          "    ROS3_PROLOGUE"]
@@ -88,7 +94,9 @@ def _function(name, params, items, off, budget, dynamic=None):
     if any(it[0] in ("lds_load", "lds_store") for it in items):
         # this wave's slice of the LDS tier: entry q of the lane at lt[q * 64]
         w.append("    double *const lt = lds_tier + (int)(threadIdx.x >> 6) * (LDS_SLOTS * kWave) + __lane_id();")
-    for it in items:
+    for k, it in enumerate(items):
+        if sched and k and k % sched == 0:
+            w.append("    __builtin_amdgcn_sched_barrier(0);")
         if it[0] == "load":
             w.append(f"    r{it[1]} = LD({addr(it[2])});")
         elif it[0] == "store":
@@ -166,6 +174,9 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     # "+ldsN": N doubles per lane of LDS as a second tier for the forward
     # functions (clean copies of evicted values that are read again)
     lds = next((int(f[3:]) for f in flags if f.startswith("lds")), 0)
+    # "+sbN": a scheduling barrier every N statements (see _function)
+    sched = next((int(f[2:]) for f in flags if f.startswith("sb")), 0)
+    fn = functools.partial(_function, sched=sched)
 
     def stage1_backward():
         # YP = Y + K1 (rbsolver.F:395-398) as soon as K1(i) is final
@@ -184,19 +195,19 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
                 _remap(solve_ops(sym, 3), scalars={"x1": "x31", "x2": "x32"}))
         items, cnt = allocate(step, {"K1", "K2", "K3"}, budget, lookahead, entry_all, lds)
         stats["ros3_step"] = _stat(step, cnt)
-        body += _function("ros3_step", ", double gdtinv, double x21, double x31, double x32", items, off, budget)
+        body += fn("ros3_step", ", double gdtinv, double x21, double x31, double x32", items, off, budget)
         b("")
     else:
         # f(y): y at a runtime workspace offset (OFF_Y or OFF_YP)
         ops = feval_ops(sym, FEVAL_ORDER)
         items, cnt = allocate(ops, {"F"}, budget, lookahead, entry_all, lds)
         stats["feval"] = _stat(ops, cnt)
-        body += _function("feval", ", int ys_off_", items, off, budget, {"YS": "ys_off_"})
+        body += fn("feval", ", int ys_off_", items, off, budget, {"YS": "ys_off_"})
         b("")
         ops = jac_lu_ops(sym, base_order, fuse_k1)
         items, cnt = allocate(ops, {"LU", "VD", "K1"} if fuse_k1 else {"LU", "VD"}, budget, lookahead, entry_all, lds)
         stats["jac_lu"] = _stat(ops, cnt)
-        body += _function("jac_lu", ", double gdtinv", items, off, budget)
+        body += fn("jac_lu", ", double gdtinv", items, off, budget)
         b("")
         for stage in (1, 2, 3):
             ops = stage1_backward() if stage == 1 else solve_ops(sym, stage)
@@ -204,13 +215,13 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
             items, cnt = allocate(ops, outs, budget, lookahead, entry_all, lds)
             stats[f"solve{stage}"] = _stat(ops, cnt)
             params = "" if stage == 1 else (", double x1" if stage == 2 else ", double x1, double x2")
-            body += _function(f"solve{stage}", params, items, off, budget)
+            body += fn(f"solve{stage}", params, items, off, budget)
             b("")
     if fuse_k1:
         # the adjoint's refactorization has no f(Y) to substitute
         ops = jac_lu_ops(sym, base_order)
         items, cnt = allocate(ops, {"LU", "VD"}, budget, lookahead, entry_all)
-        body += _function("jac_lu_plain", ", double gdtinv", items, off, budget)
+        body += fn("jac_lu_plain", ", double gdtinv", items, off, budget)
         b("")
 
     # discrete adjoint of one accepted step
@@ -224,7 +235,7 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
             ("adj_b", adj_b_ops(sym), {"LAM", "GK"}, "", None)):
         items, cnt = allocate(ops, outs, budget, lookahead, entry_all)
         stats[fname] = _stat(ops, cnt)
-        body += _function(fname, params, items, off, budget, dyn)
+        body += fn(fname, params, items, off, budget, dyn)
         b("")
 
     out = []

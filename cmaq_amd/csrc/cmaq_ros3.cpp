@@ -24,7 +24,7 @@ using namespace ros3;
 
 namespace {
 
-constexpr int kMaxRates = 640;
+constexpr int kMaxRates = 640;   // rate ops staged in LDS (80 B each); longer programs read global memory
 constexpr int kMaxMul = 64;
 constexpr int kSolveBlock = 256;
 
@@ -391,7 +391,6 @@ extern "C" int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech, c
         std::snprintf(buf, sizeof buf, "no kernels built for this mechanism (structure hash %016llx)", h);
         return set_err(nullptr, CMAQ_ROS3_E_MECH, buf);
     }
-    if (mech->nrxns > kMaxRates) return set_err(nullptr, CMAQ_ROS3_E_MECH, "more reactions than the rates kernel stages");
     cmaq_ros3_ctx *c = new cmaq_ros3_ctx();
     c->mk = mk;
     c->variants = found;
@@ -541,7 +540,10 @@ static int launch_rates(cmaq_ros3_ctx *c, int64_t n, const double *temp, const d
     P.rki = rki; P.sunlit = nullptr;
     int64_t blocks = (n + 255) / 256;
     blocks = std::min<int64_t>(blocks, 65536);
-    hipLaunchKernelGGL((rates_kernel<kMaxRates, kMaxMul>), dim3((unsigned)blocks), dim3(256), 0, s, P);
+    if (c->nr <= kMaxRates)
+        hipLaunchKernelGGL((rates_kernel<kMaxRates, kMaxMul>), dim3((unsigned)blocks), dim3(256), 0, s, P);
+    else
+        hipLaunchKernelGGL((rates_kernel<0, kMaxMul>), dim3((unsigned)blocks), dim3(256), 0, s, P);
     HIP_TRY(c, hipGetLastError());
     return 0;
 }

@@ -190,15 +190,19 @@ struct RatesParams {
 };
 
 // CALC_RCONST for every cell; the rate program is staged in LDS once per
-// workgroup and read as wave-uniform broadcasts.
+// workgroup and read as wave-uniform broadcasts.  With MAXR = 0 (programs
+// longer than the LDS stage, e.g. SAPRC07's 929 reactions) each wave reads
+// the program from global memory at the same wave-uniform addresses.
 template <int MAXR, int MAXMUL>
 __global__ __launch_bounds__(256) void rates_kernel(RatesParams P)
 {
-    __shared__ RateOp s_ops[MAXR];
+    __shared__ RateOp s_ops[MAXR > 0 ? MAXR : 1];
     __shared__ RateMul s_mul[MAXMUL];
-    for (int i = threadIdx.x; i < P.nr; i += blockDim.x) s_ops[i] = P.ops[i];
+    if (MAXR > 0)
+        for (int i = threadIdx.x; i < P.nr; i += blockDim.x) s_ops[i] = P.ops[i];
     for (int i = threadIdx.x; i < P.nmul; i += blockDim.x) s_mul[i] = P.mul[i];
     __syncthreads();
+    const RateOp *const ops = MAXR > 0 ? s_ops : P.ops;
     const int64_t n = P.ncells;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
          c += (int64_t)gridDim.x * blockDim.x) {
@@ -210,7 +214,7 @@ __global__ __launch_bounds__(256) void rates_kernel(RatesParams P)
         cell_conv(temp, pres, u);
         double *rk = P.rki + c;
         for (int r = 0; r < P.nr; ++r) {
-            const RateOp &op = s_ops[r];
+            const RateOp &op = ops[r];
             double jv = 0.0, hv = 0.0, kref = 0.0;
             if (op.kind == RK_PHOT) jv = P.rj[op.ref * n + c];
             else if (op.kind == RK_HET) hv = P.het ? P.het[op.ref * n + c] : 0.0;

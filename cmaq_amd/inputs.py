@@ -10,7 +10,7 @@ Distributions (per cell):
   every species log-uniform on [1e-6, 1e2] ppb, except O3 ~ U(20, 80) ppb and
   CO ~ U(80, 300) ppb; concentrations are passed in ppm;
   species whose chemical lifetime is shorter than the solver's minimum step
-  DTMIN = 1e-8 min (O1D in both mechanisms, ~2e-11 min) start at 1e-6 ppb:
+  DTMIN = 1e-8 min (O1D in every mechanism, ~2e-11 min) start at 1e-6 ppb:
   for them ROS3's embedded error estimate tends to y0/2 (it is not damped as
   h*lambda -> -inf), so a larger start value makes RBSOLVER reject down to
   DTMIN and stop (rbsolver.F:528-583) -- with the plain distribution the
@@ -74,6 +74,24 @@ _CLEAR_SKY_J_PER_S = {
     "MACR_RACM2": 3.0e-6, "MVK_RACM2": 5.0e-6, "GLYH2_RACM2": 1.0e-5, "GLYF_RACM2": 2.0e-5,
     "GLYHX_RACM2": 3.0e-5, "MGLY_RACM2": 1.2e-4, "BALD_RACM2": 5.0e-5, "OP1_RACM2": 5.0e-6,
     "PAA_RACM2": 1.0e-6, "ONIT_RACM2": 1.0e-6, "PAN1_RACM2": 3.0e-7, "PAN2_RACM2": 1.0e-7,
+    # CB6r3, CB6r5m, CRACMM1, RACM2 and SAPRC07 tables
+    "NO2_06": 8.0e-3, "O3O1D_06": 3.0e-5, "O3O3P_06": 4.0e-4, "NO3NO2_6": 1.8e-1,
+    "HONO_06": 1.5e-3, "HNO3": 6.0e-7, "HNO4_06": 5.0e-6, "H2O2": 7.0e-6, "PAN": 4.0e-7,
+    "HCHOR_06": 3.0e-5, "HCHOM_06": 4.5e-5, "CCHO_R": 4.0e-6, "C2CHO": 2.0e-5, "ACET_06": 5.0e-7,
+    "MEK_06": 1.0e-6, "COOH": 5.0e-6, "GLY_07R": 3.0e-5, "GLY_07M": 1.0e-5, "MGLY_06": 1.2e-4,
+    "BACL_07": 1.0e-4, "BALD_06": 5.0e-5, "AFG1": 1.0e-4, "MACR_06": 3.0e-6, "MVK_06": 5.0e-6,
+    "NOA": 1.0e-5, "PAA": 1.0e-6, "HOCCHO_IUPAC": 3.0e-6, "CL2": 2.0e-3, "CLNO_06": 2.0e-3,
+    "CLONO": 5.0e-4, "CLNO2": 3.0e-4, "HOCL_06": 2.5e-4, "CLCCHO": 1.0e-5, "CLACET": 1.0e-6,
+    "FORM_R_IUPAC10": 3.0e-5, "FORM_M_IUPAC10": 4.5e-5, "ALD2_R_IUPAC10": 4.0e-6,
+    "ALDX_R_IUPAC10": 1.5e-5, "GLYD_IUPAC10": 3.0e-6, "GLY_R_IUPAC10": 3.0e-5,
+    "HCHO_RAD_RACM2": 3.0e-5, "ALD_RACM2": 1.5e-5, "CH3COCH3_RACM2": 5.0e-7, "KET_RACM2": 1.0e-6,
+    "MEK_RACM2": 1.0e-6, "BR2_IUPAC10": 3.0e-2, "BRO_IUPAC10": 3.0e-2, "HOBR_IUPAC10": 2.0e-3,
+    "BRNO2_IUPAC10": 2.0e-3, "BRONO2_M_IUPAC10": 1.0e-4, "BRONO2_R_IUPAC10": 1.0e-3,
+    "BRCL_IUPAC10": 1.0e-2, "MB3_IUPAC10": 1.0e-7, "MB2C_BLIDE98": 1.0e-7, "MBC2_BLIDE98": 1.0e-7,
+    "COHBR_JPL2010": 1.0e-5, "I2_IUPAC10": 1.0e-1, "HOI_IUPAC10": 1.0e-2, "IO_IUPAC10": 1.0e-1,
+    "OIO_06": 1.0e-1, "INO_06": 1.0e-2, "INO2_06": 1.0e-3, "IONO2_06": 1.0e-2, "IBR_IUPAC10": 3.0e-2,
+    "ICL_IUPAC10": 1.0e-2, "CH3I_IUPAC10": 5.0e-6, "MI2_IUPAC10": 3.0e-3, "MIB_IUPAC10": 1.0e-3,
+    "MIC_IUPAC10": 3.0e-4,
 }
 
 VOC_SPECIES = {
@@ -86,8 +104,18 @@ VOC_SPECIES = {
                 "SESQ", "STY", "NAPH", "FURAN", "BDE13", "ACRO", "PROG"],
 }
 
-# species with a chemical lifetime below DTMIN at any atmospheric state
-SUB_DTMIN_SPECIES = {"CB6R5_AE7_AQ": ["O1D"], "CRACMM2": ["O1D"]}
+# species with a chemical lifetime below DTMIN (1e-8 min) somewhere in the
+# sampled atmospheric states: O(1D), loss frequency ~6.6e10 /min in every
+# mechanism of the CMAQ tree, and in CRACMM1-AMORE also ISON (up to 3.5e8 /min;
+# in the other mechanisms that carry ISON it is slow).  The next fastest,
+# O(3P), is ~1e7 /min.  Such a species starts at 1e-6 ppb instead of the
+# log-uniform draw, as it would after any earlier step.
+SUB_DTMIN_SPECIES = {"CRACMM1AMORE_AQ": ("O1D", "ISON")}
+SUB_DTMIN_DEFAULT = ("O1D",)
+
+
+def sub_dtmin_species(mech):
+    return set(SUB_DTMIN_SPECIES.get(mech["mechname"], SUB_DTMIN_DEFAULT))
 
 F_T, F_P, F_H2O, F_COSZ, F_NIGHT, F_REGIME, F_NO, F_NO2, F_SPC = 1, 2, 3, 4, 5, 6, 7, 8, 100
 
@@ -114,7 +142,7 @@ def make_cells(mech, cells, seed=0, workload="clean", nlay_cells=None, svoc_ppb=
     pres = (500.0 + 513.0 * uniform(seed, F_P, cells)) * 100.0 / 101325.0
     h2o = _logu(uniform(seed, F_H2O, cells), 1.0e3, 3.0e4)
     y = np.empty((ns, n))
-    fast = set(SUB_DTMIN_SPECIES.get(mech["mechname"], []))
+    fast = sub_dtmin_species(mech)
     for i, nm in enumerate(names):
         u = uniform(seed, F_SPC + i, cells)
         if nm in fast:
@@ -187,7 +215,7 @@ def make_cells_torch(mech, ncells, seed=0, workload="clean", nlay_cells=None, sv
         "h2o": logu(uniform_torch(seed, F_H2O, cells), 1.0e3, 3.0e4),
     }
     y = torch.empty((ns, ncells), dtype=torch.float64, device=device)
-    fast = set(SUB_DTMIN_SPECIES.get(mech["mechname"], []))
+    fast = sub_dtmin_species(mech)
     for i, nm in enumerate(names):
         u = uniform_torch(seed, F_SPC + i, cells)
         if nm in fast:

@@ -10,7 +10,7 @@ reference calls, so the expected values are the reference's own rate
 constants.  The table-driven evaluators (the oracle and the GPU kernel) are
 checked against these vectors; they never see this code.
 
-    python tests/golden/make_rate_golden.py [reference_root]
+    python tests/golden/make_rate_golden.py [reference_root [mechanism ...]]
 
 writes tests/golden/rates_<mech>.json (inputs and expected RKI).
 """
@@ -24,8 +24,15 @@ import sys
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-MECHS = {"cb6r5_ae7_aq": "CCTM/src/MECHS/cb6r5_ae7_aq/RXNS_FUNC_MODULE.F90",
-         "cracmm2": "CCTM/src/MECHS/cracmm2/RXNS_FUNC_MODULE.F90"}
+# mechanism -> cells in its fixture (fewer for the mechanisms beyond the two
+# benchmarked ones, to keep the fixtures small)
+MECHS = {"cb6r5_ae7_aq": 48, "cracmm2": 48, "cb6r3_ae7_aq": 16, "cb6r5m_ae7_aq": 16,
+         "cracmm1_aq": 16, "cracmm1amore_aq": 16, "racm2_ae6_aq": 16, "saprc07tc_ae6_aq": 16,
+         "saprc07tic_ae7i_aq": 16}
+
+
+def _source(name):
+    return f"CCTM/src/MECHS/{name}/RXNS_FUNC_MODULE.F90"
 
 
 def _statements(lines):
@@ -162,14 +169,17 @@ def main():
     sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
     from cmaq_amd.inputs import make_cells
     from cmaq_amd.ros3 import load_mechanism
-    for name, rel in MECHS.items():
+    only = sys.argv[2:]
+    for name, ncell in MECHS.items():
+        if only and name not in only:
+            continue
+        rel = _source(name)
         mech = load_mechanism(name)
         src = translate(os.path.join(ref, rel))
         # RXNS_DATA parameters used by the multiplier loops
         ns = {"math": math, "ATM_AIR": mech["atm_air"], "ATM_O2": mech["atm_o2"], "ATM_N2": mech["atm_n2"],
               "ATM_CH4": mech["atm_ch4"], "ATM_H2": mech["atm_h2"]}
         exec(compile(src, f"<{name} CALC_RCONST>", "exec"), ns)
-        ncell = 48
         d = make_cells(mech, np.arange(ncell), seed=11)
         rng = np.random.default_rng(5)
         het = rng.uniform(0.0, 1.0e-2, size=(mech["nhetero"], ncell))
