@@ -61,6 +61,7 @@ def _args():
     p.add_argument("--loop-cells", type=int, default=1048576)
     p.add_argument("--loop-sync-steps", type=int, default=12)
     p.add_argument("--no-loop", action="store_true")
+    p.add_argument("--no-chem", action="store_true", help="skip the CHEM gather/scatter leg")
     return p.parse_args()
 
 
@@ -236,6 +237,73 @@ def adjoint_leg(args, mech, solver, dev):
         "cells_without_gradient": bad}
 
 
+def chem_io_leg(args, mech, solver, d, dims, dev):
+    """CHEM-level entry points on the headline grid: the CGRID gather and
+    scatter kernels (HBM-bound copies with unit conversion) and one whole
+    cmaq_ros3_chem_device call."""
+    import torch
+    ncol, nrow, nlay = dims
+    n = ncol * nrow * nlay
+    ns, nph, nhet = mech["numb_mech_spc"], mech["nphotab"], mech["nhetero"]
+    mwair, mwwat, rgas = 28.9628, 18.0153, 8.314510
+    g = torch.Generator(device="cpu").manual_seed(args.seed + 5)
+    nspcsd = ns + 64
+    idx = (torch.randperm(nspcsd, generator=g)[:ns] + 1).to(torch.int32)
+    conv = np.array([1 if c else 0 for c in mech["convert_conc"]], np.int32)
+    mw = np.array(mech["species_molwt"], np.float64)
+    solver.set_cgrid_map(nspcsd, idx.numpy(), conv, mw)
+    pres_pa = d["pres"] * 101325.0
+    dens = pres_pa * mwair * 1.0e-3 / (rgas * d["temp"])
+    to_ug = torch.as_tensor(mw, device=dev)[:, None] * dens[None, :] / (1.0e-3 * mwair)
+    cgrid = torch.full((nspcsd, n), -1.0, dtype=torch.float32, device=dev)
+    convd = torch.as_tensor(conv, device=dev)[:, None] == 1
+    cgrid[idx.to(dev).long() - 1] = torch.where(convd, d["y"] * to_ug, d["y"]).float()
+    del to_ug, convd
+    grid = {"ncols": ncol, "nrows": nrow, "nlays": nlay, "cgrid": cgrid,
+            "ta": d["temp"].float(), "dens": dens.float(), "pres": pres_pa.float(),
+            "qv": (d["h2o"] * 1.0e-6 * mwwat / mwair).float(),
+            "seawater_zone": torch.rand(ncol * nrow, generator=g).to(dev),
+            "seaice": (0.2 * torch.rand(ncol * nrow, generator=g)).to(dev),
+            "rj": d["rj"].float(),
+            "khetero": (1e-3 * torch.rand((n, nhet + 1), generator=g, dtype=torch.float64)).to(dev)}
+    del dens, pres_pa
+    blk = solver.gather_device(grid)
+    torch.cuda.synchronize()
+    reps = 5
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record()
+    for _ in range(reps):
+        solver.gather_device(grid)
+    ev[1].record()
+    for _ in range(reps):
+        solver.scatter_device(grid, blk["y"])
+    ev[2].record()
+    torch.cuda.synchronize()
+    g_ms = ev[0].elapsed_time(ev[1]) / reps
+    s_ms = ev[1].elapsed_time(ev[2]) / reps
+    del blk
+    # algorithmic bytes per cell: every input read once, every output written once
+    b_gather = 4 * 4 + 4 * ns + 4 * nph + 8 * nhet + 8.0 / nlay + 8 * (4 + ns + nph + nhet)
+    b_scatter = 4 + 8 * ns + 4 * ns
+    gb_g = b_gather * n / (g_ms / 1e3) / 1e9
+    gb_s = b_scatter * n / (s_ms / 1e3) / 1e9
+    na = torch.zeros(n, dtype=torch.int32, device=dev)
+    nj = torch.zeros(n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.chem_device(grid, 5.0, nacc=na, nrej=nj, raise_on_fail=False)
+    torch.cuda.synchronize()
+    chem_s = time.perf_counter() - t0
+    att = int((na.clamp(min=0) + nj).sum())
+    return {"grid": f"{ncol}x{nrow}x{nlay}", "ncells": n, "nspcsd": nspcsd,
+            "gather_kernel_ms": g_ms, "scatter_kernel_ms": s_ms,
+            "gather_bytes_per_cell": b_gather, "scatter_bytes_per_cell": b_scatter,
+            "gather_GBps": gb_g, "scatter_GBps": gb_s,
+            "gather_frac_of_hbm_peak": gb_g / HBM_PEAK_GBS, "scatter_frac_of_hbm_peak": gb_s / HBM_PEAK_GBS,
+            "chem_device_ms": chem_s * 1e3, "chem_device_attempted_cell_steps_per_s": att / chem_s,
+            "chem_device_failed_cells": int((na < 0).sum())}
+
+
 def main():
     args = _args()
     if args.gpus != 1:
@@ -343,6 +411,13 @@ def main():
                  "frac": fp64_tfs / FP64_PEAK_TFS, "flops_per_attempted_step": f_step},
         "fwd_adj_ratio": None,
     }
+    if not args.no_chem and args.workload == "regional" and not args.ncells:
+        print("[bench] CHEM gather/scatter leg", file=sys.stderr, flush=True)
+        del y
+        torch.cuda.empty_cache()
+        out["chem_io"] = chem_io_leg(args, mech, solver, d, (ncol, nrow, nlay), dev)
+        y = None
+        torch.cuda.empty_cache()
     if not (args.no_adj and args.no_loop):
         del y, y0, d, na, nj, sunlit
         torch.cuda.empty_cache()

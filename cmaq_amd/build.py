@@ -137,6 +137,8 @@ def _compile(src, obj, host_only=False, extra=(), force=False):
     same_cmd = os.path.exists(stamp) and open(stamp).read() == " ".join(cmd)
     deps = [src, os.path.join(CSRC, "ros3_device.h"), os.path.join(CSRC, "ros3_registry.h"), os.path.join(CSRC, "ros3_solver_kernel.inc"), os.path.join(CSRC, "ros3_adjoint_kernel.inc"),
             os.path.join(CSRC, "ros3_registry.h"), os.path.join(HERE, "..", "include", "cmaq_ros3.h")]
+    if src.endswith("cmaq_ros3.cpp"):
+        deps.append(os.path.join(CSRC, "ros3_chem_io.h"))
     if not force and same_cmd and os.path.exists(obj) and \
             all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps if os.path.exists(d)):
         return obj

@@ -19,6 +19,7 @@
 
 #include "../../include/cmaq_ros3.h"
 #include "ros3_device.h"
+#include "ros3_chem_io.h"
 
 using namespace ros3;
 
@@ -85,6 +86,13 @@ struct cmaq_ros3_ctx {
     int32_t *d_step_na = nullptr, *d_step_nr = nullptr;
     int64_t step_cells = 0;
     double steps_ms = 0.0;
+    // CHEM level: CGRID species map and the block arrays of all cells
+    ChemMap *d_map = nullptr;
+    int32_t nspcsd = 0;
+    void *d_block = nullptr;      // temp, pres, h2o, sea, rj, het, y
+    size_t block_bytes = 0;
+    void *d_grid = nullptr;       // host cmaq_ros3_chem: the grid's arrays on the device
+    size_t grid_bytes = 0;
     std::string err;
 };
 
@@ -465,6 +473,7 @@ extern "C" void cmaq_ros3_free(cmaq_ros3_ctx *c)
     hipFree(c->d_ws); hipFree(c->d_small); hipFree(c->d_rki); hipFree(c->d_io);
     hipFree(c->d_tape); hipFree(c->d_tape_off); hipFree(c->d_order);
     hipFree(c->d_step_na); hipFree(c->d_step_nr);
+    hipFree(c->d_map); hipFree(c->d_block); hipFree(c->d_grid);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
     if (c->gstream) hipStreamDestroy(c->gstream);
@@ -901,6 +910,7 @@ extern "C" int cmaq_ros3_run_steps_device(cmaq_ros3_ctx *c, int64_t n, double ch
     }
     if (c->step_cells < n) {
         hipFree(c->d_step_na); hipFree(c->d_step_nr);
+    hipFree(c->d_map); hipFree(c->d_block); hipFree(c->d_grid);
         c->d_step_na = c->d_step_nr = nullptr;
         c->step_cells = 0;
         if (hipMalloc((void **)&c->d_step_na, (size_t)n * 4) != hipSuccess ||
@@ -965,4 +975,189 @@ extern "C" int cmaq_ros3_last_steps_timing(const cmaq_ros3_ctx *c, double *ms)
     if (!c || !ms) return CMAQ_ROS3_E_ARG;
     *ms = c->steps_ms;
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// CHEM level: CGRID gather / scatter around the solve (rbdriver.F:460-595)
+// ---------------------------------------------------------------------------
+namespace {
+
+// grow-only device buffer
+int ensure_buffer(cmaq_ros3_ctx *c, void **buf, size_t *have, size_t need, const char *what)
+{
+    if (*have >= need) return 0;
+    hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+    if (hipMalloc(buf, need) != hipSuccess)
+        return set_err(c, CMAQ_ROS3_E_NOMEM, std::string("hipMalloc of ") + what + " failed");
+    *have = need;
+    return 0;
+}
+
+int check_grid(cmaq_ros3_ctx *c, const cmaq_ros3_grid *g)
+{
+    if (!c->d_map) return set_err(c, CMAQ_ROS3_E_ARG, "cmaq_ros3_set_cgrid_map has not been called");
+    if (!g || g->ncols <= 0 || g->nrows <= 0 || g->nlays <= 0)
+        return set_err(c, CMAQ_ROS3_E_ARG, "bad grid dimensions");
+    if (!g->cgrid || !g->ta || !g->dens || !g->pres || !g->qv || (c->nphotab > 0 && !g->rj))
+        return set_err(c, CMAQ_ROS3_E_ARG, "missing grid array");
+    if (g->khetero && g->ld_het < c->nhetero) return set_err(c, CMAQ_ROS3_E_ARG, "ld_het < NHETERO");
+    return 0;
+}
+
+ChemParams chem_params(const cmaq_ros3_ctx *c, const cmaq_ros3_grid *g)
+{
+    ChemParams P;
+    std::memset(&P, 0, sizeof P);
+    P.plane2 = (int64_t)g->ncols * g->nrows;
+    P.n = P.plane2 * g->nlays;
+    P.ns = c->ns; P.nphotab = c->nphotab; P.nhetero = c->nhetero; P.ld_het = g->ld_het;
+    P.map = c->d_map;
+    P.cgrid = g->cgrid; P.ta = g->ta; P.dens = g->dens; P.pres = g->pres; P.qv = g->qv;
+    P.zone = g->seawater_zone; P.ice = g->seaice; P.rj = g->rj; P.khet = g->khetero;
+    return P;
+}
+
+unsigned chem_blocks(int64_t n)
+{
+    return (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+}
+
+}  // namespace
+
+extern "C" int cmaq_ros3_set_cgrid_map(cmaq_ros3_ctx *c, int32_t nspcsd, const int32_t *cgrid_index,
+                                       const int32_t *convert_conc, const double *species_molwt)
+{
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (nspcsd <= 0 || !cgrid_index || !convert_conc || !species_molwt)
+        return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (c->ns > kMaxChemSpc) return set_err(c, CMAQ_ROS3_E_MECH, "more species than the gather kernel stages");
+    std::vector<ChemMap> map(c->ns);
+    std::vector<char> seen(nspcsd, 0);
+    for (int i = 0; i < c->ns; ++i) {
+        const int32_t k = cgrid_index[i];
+        if (k < 1 || k > nspcsd || seen[k - 1])
+            return set_err(c, CMAQ_ROS3_E_ARG, "CGRID index out of range or repeated for species " + std::to_string(i + 1));
+        seen[k - 1] = 1;
+        map[i].spc = k - 1;
+        map[i].conv = convert_conc[i] ? 1 : 0;
+        // rbinit.F:260-261: REAL(4) arithmetic, REVERSE_CONV widened to REAL(8)
+        const float num = 1.0e-3f * kMwAir;
+        const float fwd = num / (float)species_molwt[i];
+        map[i].fwd = fwd;
+        map[i].pad = 0.0f;
+        map[i].rev = (double)(1.0f / fwd);
+        if (map[i].conv && !(species_molwt[i] > 0.0))
+            return set_err(c, CMAQ_ROS3_E_ARG, "SPECIES_MOLWT <= 0 for a converted species");
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->d_map && hipMalloc((void **)&c->d_map, kMaxChemSpc * sizeof(ChemMap)) != hipSuccess)
+        return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of the species map failed");
+    HIP_TRY(c, hipMemcpy(c->d_map, map.data(), map.size() * sizeof(ChemMap), hipMemcpyHostToDevice));
+    c->nspcsd = nspcsd;
+    return 0;
+}
+
+extern "C" int cmaq_ros3_gather_device(cmaq_ros3_ctx *c, const cmaq_ros3_grid *g, double *temp,
+                                       double *pres_atm, double *h2o, double *rj, double *het,
+                                       double *sea, double *y, void *stream)
+{
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (int rc = check_grid(c, g)) return rc;
+    if (!temp || !pres_atm || !h2o || !sea || !y || (c->nphotab > 0 && !rj))
+        return set_err(c, CMAQ_ROS3_E_ARG, "missing block array");
+    HIP_TRY(c, hipSetDevice(c->device));
+    ChemParams P = chem_params(c, g);
+    P.temp = temp; P.pres_atm = pres_atm; P.h2o = h2o; P.sea = sea; P.rjd = rj; P.het = het; P.y = y;
+    hipLaunchKernelGGL(gather_kernel, dim3(chem_blocks(P.n)), dim3(256), 0, (hipStream_t)stream, P);
+    HIP_TRY(c, hipGetLastError());
+    return 0;
+}
+
+extern "C" int cmaq_ros3_scatter_device(cmaq_ros3_ctx *c, const cmaq_ros3_grid *g, const double *y,
+                                        void *stream)
+{
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (int rc = check_grid(c, g)) return rc;
+    if (!y) return set_err(c, CMAQ_ROS3_E_ARG, "missing y");
+    HIP_TRY(c, hipSetDevice(c->device));
+    ChemParams P = chem_params(c, g);
+    P.y = const_cast<double *>(y);
+    hipLaunchKernelGGL(scatter_kernel, dim3(chem_blocks(P.n)), dim3(256), 0, (hipStream_t)stream, P);
+    HIP_TRY(c, hipGetLastError());
+    return 0;
+}
+
+extern "C" int cmaq_ros3_chem_device(cmaq_ros3_ctx *c, const cmaq_ros3_grid *g, double chemstep,
+                                     int32_t *nacc, int32_t *nrej, cmaq_ros3_fail *fail, void *stream)
+{
+    if (fail) { fail->code = 0; fail->pad = 0; fail->cell = -1; fail->maxerr = fail->dt = fail->tnow = 0.0; }
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (int rc = check_grid(c, g)) return rc;
+    if (!(chemstep > 0.0)) return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t n = (int64_t)g->ncols * g->nrows * g->nlays;
+    const size_t nd = (size_t)n * sizeof(double);
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t sizes[7] = {nd, nd, nd, nd, nd * c->nphotab, g->khetero ? nd * c->nhetero : 0, nd * c->ns};
+    size_t need = 0;
+    for (size_t b : sizes) need += al(b);
+    if (int rc = ensure_buffer(c, &c->d_block, &c->block_bytes, need, "CHEM block arrays")) return rc;
+    double *b[7];
+    char *p = (char *)c->d_block;
+    for (int i = 0; i < 7; ++i) { b[i] = sizes[i] ? (double *)p : nullptr; p += al(sizes[i]); }
+    if (int rc = cmaq_ros3_gather_device(c, g, b[0], b[1], b[2], b[4], b[5], b[3], b[6], stream)) return rc;
+    int rc = cmaq_ros3_run_device(c, n, chemstep, b[0], b[1], b[2], b[4], b[5], b[3], b[6], nacc, nrej,
+                                  fail, stream);
+    if (rc != 0 && rc != CMAQ_ROS3_E_CONVERGENCE && rc != CMAQ_ROS3_E_STEPS) return rc;
+    if (int rs = cmaq_ros3_scatter_device(c, g, b[6], stream)) return rs;
+    HIP_TRY(c, hipStreamSynchronize((hipStream_t)stream));
+    return rc;
+}
+
+extern "C" int cmaq_ros3_chem(cmaq_ros3_ctx *c, const cmaq_ros3_grid *g, double chemstep, int32_t *nacc,
+                              int32_t *nrej, cmaq_ros3_fail *fail)
+{
+    if (fail) { fail->code = 0; fail->pad = 0; fail->cell = -1; fail->maxerr = fail->dt = fail->tnow = 0.0; }
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (int rc = check_grid(c, g)) return rc;
+    if (!(chemstep > 0.0)) return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t plane2 = (int64_t)g->ncols * g->nrows, n = plane2 * g->nlays;
+    const size_t nf = (size_t)n * sizeof(float), n2 = (size_t)plane2 * sizeof(float);
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    // cgrid planes, ta, dens, pres, qv, zone, ice, rj, khetero, nacc, nrej
+    const size_t sizes[11] = {nf * c->nspcsd, nf, nf, nf, nf, g->seawater_zone ? n2 : 0, g->seaice ? n2 : 0,
+                              nf * c->nphotab,
+                              g->khetero ? (size_t)n * g->ld_het * sizeof(double) : 0, (size_t)n * 4,
+                              (size_t)n * 4};
+    size_t need = 0;
+    for (size_t b : sizes) need += al(b);
+    if (int rc = ensure_buffer(c, &c->d_grid, &c->grid_bytes, need, "CHEM grid arrays")) return rc;
+    char *b[11];
+    char *p = (char *)c->d_grid;
+    for (int i = 0; i < 11; ++i) { b[i] = sizes[i] ? p : nullptr; p += al(sizes[i]); }
+    std::vector<ChemMap> map(c->ns);
+    HIP_TRY(c, hipMemcpy(map.data(), c->d_map, map.size() * sizeof(ChemMap), hipMemcpyDeviceToHost));
+    // only the mechanism species' planes travel
+    for (const ChemMap &m : map)
+        HIP_TRY(c, hipMemcpy(b[0] + (size_t)m.spc * nf, g->cgrid + (size_t)m.spc * n, nf, hipMemcpyHostToDevice));
+    const void *src[8] = {nullptr, g->ta, g->dens, g->pres, g->qv, g->seawater_zone, g->seaice, g->rj};
+    for (int i = 1; i < 8; ++i)
+        if (sizes[i] && src[i]) HIP_TRY(c, hipMemcpy(b[i], src[i], sizes[i], hipMemcpyHostToDevice));
+    if (g->khetero) HIP_TRY(c, hipMemcpy(b[8], g->khetero, sizes[8], hipMemcpyHostToDevice));
+    cmaq_ros3_grid dg = *g;
+    dg.cgrid = (float *)b[0];
+    dg.ta = (const float *)b[1]; dg.dens = (const float *)b[2]; dg.pres = (const float *)b[3];
+    dg.qv = (const float *)b[4];
+    dg.seawater_zone = (const float *)b[5]; dg.seaice = (const float *)b[6];
+    dg.rj = (const float *)b[7]; dg.khetero = (const double *)b[8];
+    int rc = cmaq_ros3_chem_device(c, &dg, chemstep, (int32_t *)b[9], (int32_t *)b[10], fail, nullptr);
+    if (rc != 0 && rc != CMAQ_ROS3_E_CONVERGENCE && rc != CMAQ_ROS3_E_STEPS) return rc;
+    for (const ChemMap &m : map)
+        HIP_TRY(c, hipMemcpy(g->cgrid + (size_t)m.spc * n, b[0] + (size_t)m.spc * nf, nf, hipMemcpyDeviceToHost));
+    if (nacc) HIP_TRY(c, hipMemcpy(nacc, b[9], (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (nrej) HIP_TRY(c, hipMemcpy(nrej, b[10], (size_t)n * 4, hipMemcpyDeviceToHost));
+    return rc;
 }

@@ -253,3 +253,51 @@ def make_cells_torch(mech, ncells, seed=0, workload="clean", nlay_cells=None, sv
     out["y"] = y
     out["night"] = night
     return out
+
+
+# ---- CCTM-side arrays of one CHEM call -----------------------------------
+F_MAP, F_ZONE, F_ICE, F_HET = 9, 10, 11, 12
+_MWAIR, _MWWAT, _RGAS = 28.9628, 18.0153, 8.314510   # CONST.EXT: MWAIR, MWWAT, RGASUNIV
+
+
+def make_grid(mech, ncols, nrows, nlays, seed=0, workload="clean", extra_species=64, ld_het_pad=1):
+    """The REAL(4) arrays CHEM holds for a (ncols, nrows, nlays) grid, from the
+    same per-cell draws as make_cells (cell = col + ncols*(row + nrows*lay)).
+
+    CGRID has the mechanism species at a seeded permutation of CGRID indices
+    among numb_mech_spc + extra_species planes (the other planes hold a
+    sentinel); aerosol species (CONVERT_CONC) are stored in ug/m3 and gas
+    species in ppm.  DENS = P*MWAIR/(R*T); QV from the H2O mixing ratio;
+    SEAWATER_ZONE and SEAICE are seeded 2-D fields; KHETERO is
+    [cell][nhetero + ld_het_pad] with seeded rates.  Returns (grid dict,
+    cgrid_index (1-based), convert_conc, species_molwt, cell dict of
+    make_cells)."""
+    n = ncols * nrows * nlays
+    cells = np.arange(n, dtype=np.int64)
+    d = make_cells(mech, cells, seed=seed, workload=workload, nlay_cells=ncols * nrows)
+    ns = mech["numb_mech_spc"]
+    nspcsd = ns + extra_species
+    perm = np.argsort(uniform(seed, F_MAP, np.arange(nspcsd, dtype=np.int64)))
+    cgrid_index = (perm[:ns] + 1).astype(np.int32)
+    conv = np.array([1 if c else 0 for c in mech["convert_conc"]], np.int32)
+    molwt = np.array(mech["species_molwt"], np.float64)
+    pres_pa = d["pres"] * 101325.0
+    dens = pres_pa * _MWAIR * 1.0e-3 / (_RGAS * d["temp"])
+    qv = d["h2o"] * 1.0e-6 * _MWWAT / _MWAIR
+    cgrid = np.full((nspcsd, n), -1.0, np.float32)
+    to_ug = molwt[:, None] * dens[None, :] / (1.0e-3 * _MWAIR)     # ppm -> ug/m3
+    cgrid[cgrid_index - 1] = np.where(conv[:, None] == 1, d["y"] * to_ug, d["y"]).astype(np.float32)
+    c2 = np.arange(ncols * nrows, dtype=np.int64)
+    zone = np.where(uniform(seed, F_ZONE, c2) < 0.5, 0.0, uniform(seed, F_ZONE + 100, c2))
+    ice = np.where(uniform(seed, F_ICE, c2) < 0.8, 0.0, 0.5 * uniform(seed, F_ICE + 100, c2))
+    nhet = mech["nhetero"]
+    ld = nhet + ld_het_pad
+    kh = np.zeros((n, ld))
+    for k in range(nhet):
+        kh[:, k] = _logu(uniform(seed, F_HET + k, cells), 1.0e-6, 1.0e-1)
+    grid = {"ncols": ncols, "nrows": nrows, "nlays": nlays, "nspcsd": nspcsd,
+            "cgrid": cgrid, "ta": d["temp"].astype(np.float32), "dens": dens.astype(np.float32),
+            "pres": pres_pa.astype(np.float32), "qv": qv.astype(np.float32),
+            "seawater_zone": zone.astype(np.float32), "seaice": ice.astype(np.float32),
+            "rj": d["rj"].astype(np.float32), "khetero": kh}
+    return grid, cgrid_index, conv, molwt, d

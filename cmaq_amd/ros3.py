@@ -59,6 +59,13 @@ class _Fail(C.Structure):
                 ("dt", C.c_double), ("tnow", C.c_double)]
 
 
+class _Grid(C.Structure):
+    _fields_ = [("ncols", C.c_int32), ("nrows", C.c_int32), ("nlays", C.c_int32), ("ld_het", C.c_int32),
+                ("cgrid", C.c_void_p), ("ta", C.c_void_p), ("dens", C.c_void_p), ("pres", C.c_void_p),
+                ("qv", C.c_void_p), ("seawater_zone", C.c_void_p), ("seaice", C.c_void_p),
+                ("rj", C.c_void_p), ("khetero", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -97,6 +104,11 @@ def lib():
         L.cmaq_ros3_set_variant.argtypes = [vp, C.c_char_p]
         L.cmaq_ros3_variant.argtypes = [vp, C.c_int]
         L.cmaq_ros3_variant.restype = C.c_char_p
+        L.cmaq_ros3_set_cgrid_map.argtypes = [vp, C.c_int32, _i32p, _i32p, _f64p]
+        L.cmaq_ros3_chem.argtypes = [vp, C.POINTER(_Grid), C.c_double, _i32p, _i32p, C.POINTER(_Fail)]
+        L.cmaq_ros3_chem_device.argtypes = [vp, C.POINTER(_Grid), C.c_double, vp, vp, C.POINTER(_Fail), vp]
+        L.cmaq_ros3_gather_device.argtypes = [vp, C.POINTER(_Grid)] + [vp] * 7 + [vp]
+        L.cmaq_ros3_scatter_device.argtypes = [vp, C.POINTER(_Grid), vp, vp]
         _lib = L
     return _lib
 
@@ -272,6 +284,103 @@ class Ros3Solver:
         if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
             raise self._err(rc, fail)
         return fail if rc != 0 else None
+
+    # ---- CHEM level (CGRID in, CGRID out) ----
+    def set_cgrid_map(self, nspcsd, cgrid_index, convert_conc, species_molwt):
+        """MAP_CHEMISTRY_SPECIES result: 1-based CGRID index, CONVERT_CONC and
+        SPECIES_MOLWT per mechanism species."""
+        idx = np.ascontiguousarray(cgrid_index, np.int32)
+        conv = np.ascontiguousarray(convert_conc, np.int32)
+        mw = np.ascontiguousarray(species_molwt, np.float64)
+        if idx.size != self.ns or conv.size != self.ns or mw.size != self.ns:
+            raise ValueError("one entry per mechanism species expected")
+        rc = lib().cmaq_ros3_set_cgrid_map(self._ctx, int(nspcsd), idx.ctypes.data_as(_i32p),
+                                           conv.ctypes.data_as(_i32p), mw.ctypes.data_as(_f64p))
+        if rc != 0:
+            raise self._err(rc)
+
+    def _grid(self, g, device):
+        """cmaq_ros3_grid over numpy arrays (host) or torch tensors (device)."""
+        if device:
+            import torch
+        want = {"cgrid": np.float32, "ta": np.float32, "dens": np.float32, "pres": np.float32,
+                "qv": np.float32, "seawater_zone": np.float32, "seaice": np.float32, "rj": np.float32,
+                "khetero": np.float64}
+        st = _Grid()
+        st.ncols, st.nrows, st.nlays = int(g["ncols"]), int(g["nrows"]), int(g["nlays"])
+        n = st.ncols * st.nrows * st.nlays
+        for k, dt in want.items():
+            a = g.get(k)
+            if a is None:
+                setattr(st, k, None)
+                continue
+            if device:
+                tdt = torch.float32 if dt == np.float32 else torch.float64
+                if not (a.is_cuda and a.is_contiguous() and a.dtype == tdt):
+                    raise ValueError(f"{k} must be a contiguous {tdt} CUDA tensor")
+                setattr(st, k, a.data_ptr())
+            else:
+                if not (isinstance(a, np.ndarray) and a.dtype == dt and a.flags["C_CONTIGUOUS"]):
+                    raise ValueError(f"{k} must be a C-contiguous {np.dtype(dt).name} array")
+                setattr(st, k, a.ctypes.data)
+        kh = g.get("khetero")
+        st.ld_het = (kh.numel() if device else kh.size) // n if kh is not None else self.nhetero
+        return st
+
+    def chem(self, grid, chemstep, raise_on_fail=True):
+        """One CHEM call on host arrays (make_grid's dict of numpy arrays):
+        grid["cgrid"] is updated in place; returns (n_accepted, n_rejected)."""
+        st = self._grid(grid, device=False)
+        n = st.ncols * st.nrows * st.nlays
+        na, nj = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        f = _Fail()
+        rc = lib().cmaq_ros3_chem(self._ctx, C.byref(st), chemstep, na.ctypes.data_as(_i32p),
+                                  nj.ctypes.data_as(_i32p), C.byref(f))
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, {"cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow})
+        return na, nj
+
+    def chem_device(self, grid, chemstep, nacc=None, nrej=None, stream=None, raise_on_fail=True):
+        """One CHEM call on device arrays (the same dict with torch CUDA tensors)."""
+        import torch
+        st = self._grid(grid, device=True)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_ros3_chem_device(self._ctx, C.byref(st), chemstep, _ptr(nacc), _ptr(nrej), C.byref(f),
+                                         C.c_void_p(stream))
+        fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, fail)
+        return fail if rc != 0 else None
+
+    def gather_device(self, grid, stream=None):
+        """The block arrays of every cell (dict of float64 CUDA tensors)."""
+        import torch
+        st = self._grid(grid, device=True)
+        n = st.ncols * st.nrows * st.nlays
+        dev = grid["ta"].device
+        z = lambda r: torch.zeros((r, n), dtype=torch.float64, device=dev)
+        out = {"temp": z(1)[0], "pres_atm": z(1)[0], "h2o": z(1)[0], "sea": z(1)[0], "rj": z(self.nphotab),
+               "het": z(self.nhetero) if grid.get("khetero") is not None else None, "y": z(self.ns)}
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        rc = lib().cmaq_ros3_gather_device(self._ctx, C.byref(st), _ptr(out["temp"]), _ptr(out["pres_atm"]),
+                                           _ptr(out["h2o"]), _ptr(out["rj"]), _ptr(out["het"]), _ptr(out["sea"]),
+                                           _ptr(out["y"]), C.c_void_p(stream))
+        if rc != 0:
+            raise self._err(rc)
+        return out
+
+    def scatter_device(self, grid, y, stream=None):
+        import torch
+        st = self._grid(grid, device=True)
+        self._check([("y", y, self.ns)], st.ncols * st.nrows * st.nlays)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        rc = lib().cmaq_ros3_scatter_device(self._ctx, C.byref(st), _ptr(y), C.c_void_p(stream))
+        if rc != 0:
+            raise self._err(rc)
 
     def _check(self, tensors, n):
         import torch

@@ -222,6 +222,77 @@ int cmaq_ros3_rates_device(cmaq_ros3_ctx *ctx, int64_t ncells,
  */
 int cmaq_ros3_last_timing(const cmaq_ros3_ctx *ctx, double *rates_ms, double *solve_ms);
 
+/* ---- CHEM level: CGRID in, CGRID out ----------------------------------- */
+
+/*
+ * The arrays of one CHEM call (rbdriver.F:20) as CCTM holds them, Fortran
+ * column-major REAL(4) unless noted; "grid" = (NCOLS,NROWS,NLAYS):
+ *   cgrid          CONC(NCOLS,NROWS,NLAYS,NSPCSD), in/out (rbdriver.F:100):
+ *                  gas ppm, aerosol species ug/m3 (converted as CONVERT_CONC)
+ *   ta, dens,      TA (K), DENS (kg/m3), PRES (Pa), QV (kg/kg) on the grid
+ *   pres, qv       (rbdriver.F:169-172, interpolated by CHEM at :390-403)
+ *   seawater_zone  SEAWATER_ZONE(NCOLS,NROWS) = OCEAN + SZONE (rbdriver.F:253)
+ *   seaice         SEAICE(NCOLS,NROWS) (rbdriver.F:390); either may be NULL = 0
+ *   rj             RJ(NCOLS,NROWS,NLAYS,NPHOTAB) 1/min (PHOT_MOD.F:109)
+ *   khetero        REAL(8) KHETERO(ld_het,NCOLS,NROWS,NLAYS) 1/min
+ *                  (AEROSOL_CHEMISTRY.F:46, 712), ld_het >= NHETERO; NULL = 0
+ * Cells are integrated in CGRID's own order, cell = col + ncols*(row +
+ * nrows*lay); results do not depend on the order (per-cell step control).
+ */
+typedef struct cmaq_ros3_grid {
+    int32_t ncols, nrows, nlays, ld_het;
+    float *cgrid;
+    const float *ta, *dens, *pres, *qv;
+    const float *seawater_zone, *seaice;
+    const float *rj;
+    const double *khetero;
+} cmaq_ros3_grid;
+
+/*
+ * Species map of MAP_CHEMISTRY_SPECIES / RESET_SPECIES_POINTERS
+ * (RXNS_FUNC_MODULE.F90:1117-1305) and RBINIT's conversion factors
+ * (rbinit.F:257-262), per mechanism species in CHEMISTRY_SPC order:
+ * cgrid_index (1-based CGRID species, <= nspcsd, all distinct),
+ * convert_conc (nonzero = CGRID holds ug/m3: FORWARD_CONV = 1e-3*MWAIR/MOLWT
+ * in REAL(4)) and species_molwt (RXNS_DATA SPECIES_MOLWT, g/mol).  Required
+ * before the calls below.
+ */
+int cmaq_ros3_set_cgrid_map(cmaq_ros3_ctx *ctx, int32_t nspcsd, const int32_t *cgrid_index,
+                            const int32_t *convert_conc, const double *species_molwt);
+
+/*
+ * The whole gas-chemistry part of CHEM for every cell of the grid: the
+ * gather with unit conversion (rbdriver.F:460-509), CALC_RCONST + RBSOLVER
+ * (as cmaq_ros3_run_device) and the scatter back into CGRID
+ * (rbdriver.F:577-595).  Host arrays; the CGRID planes of the mechanism
+ * species, the met fields, RJ and KHETERO are copied to the device and the
+ * planes back.  n_accepted / n_rejected [ncells] (may be NULL), failures as
+ * in cmaq_ros3_run (a failing cell's CGRID holds its last accepted state).
+ */
+int cmaq_ros3_chem(cmaq_ros3_ctx *ctx, const cmaq_ros3_grid *grid, double chemstep_min,
+                   int32_t *n_accepted, int32_t *n_rejected, cmaq_ros3_fail *fail);
+
+/* Same as cmaq_ros3_chem with every array of *grid and the counts in device
+ * memory, enqueued on `stream`; returns after the stream has finished. */
+int cmaq_ros3_chem_device(cmaq_ros3_ctx *ctx, const cmaq_ros3_grid *grid, double chemstep_min,
+                          int32_t *n_accepted, int32_t *n_rejected, cmaq_ros3_fail *fail,
+                          void *stream);
+
+/*
+ * The two halves of cmaq_ros3_chem_device on device arrays.  gather fills the
+ * block arrays of rbdriver.F:460-509 for every cell -- temp_k, pres_atm,
+ * h2o_ppm, seawater [ncells]; rj_per_min [nphotab][ncells]; het_per_min
+ * [nhetero][ncells] (may be NULL); y_ppm [numb_mech_spc][ncells] in
+ * mechanism order -- in the reference's REAL(4) arithmetic, bit for bit.
+ * scatter writes y_ppm back into grid->cgrid (rbdriver.F:577-595), reading
+ * grid->dens.  Enqueued on `stream`, asynchronous.
+ */
+int cmaq_ros3_gather_device(cmaq_ros3_ctx *ctx, const cmaq_ros3_grid *grid, double *temp_k,
+                            double *pres_atm, double *h2o_ppm, double *rj_per_min,
+                            double *het_per_min, double *seawater, double *y_ppm, void *stream);
+int cmaq_ros3_scatter_device(cmaq_ros3_ctx *ctx, const cmaq_ros3_grid *grid, const double *y_ppm,
+                             void *stream);
+
 /*
  * Kernel variants: every mechanism is generated in several register-budget /
  * load-lookahead configurations (chosen per mechanism by the autotuner and

@@ -82,6 +82,10 @@ def lib():
         L.om_adjoint_run.argtypes = [vp, C.c_int64, C.c_double] + [_f64p] * 7 + [_i32p]
         L.om_ddm_cell.argtypes = [vp, C.c_double, _f64p, _f64p, _f64p, _f64p, C.c_int, _i32p,
                                   C.POINTER(C.c_float)]
+        _f32p = C.POINTER(C.c_float)
+        L.om_gather.argtypes = [C.c_int] * 6 + [_i32p, _i32p, _f64p] + [_f32p] * 8 + [_f64p, C.c_int] + \
+            [_f64p] * 7
+        L.om_scatter.argtypes = [C.c_int] * 4 + [_i32p, _i32p, _f64p, _f32p, _f64p, _f32p]
         _lib = L
     return _lib
 
@@ -328,3 +332,59 @@ class Oracle:
             return s_old[:ns].astype(np.float64), s_old[ns:].astype(np.float64), yend
         wv = np.asarray(w, np.float64)
         return s_old[:ns].astype(np.float64) @ wv, s_old[ns:].astype(np.float64) @ wv, yend
+
+
+# ---- CHEM's CGRID gather / scatter (rbdriver.F:460-509, 577-595) ----
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def chem_gather(mech, grid, cgrid_index, convert_conc, molwt):
+    """Block arrays of every cell from CGRID and the met fields.  grid: dict
+    with ncols, nrows, nlays, cgrid [nspcsd][n] and ta, dens, pres, qv [n],
+    rj [nphotab][n] (float32, cell = col + ncols*(row + nrows*lay)); optional
+    seawater_zone, seaice [ncols*nrows] (float32), khetero [n][ld_het]
+    (float64).  Returns a dict of float64 arrays; y in mechanism order."""
+    nc, nr_, nl = grid["ncols"], grid["nrows"], grid["nlays"]
+    n = nc * nr_ * nl
+    ns, nph, nhet = mech["numb_mech_spc"], mech["nphotab"], mech["nhetero"]
+    idx = np.ascontiguousarray(cgrid_index, np.int32)
+    conv = np.ascontiguousarray(convert_conc, np.int32)
+    mw = np.ascontiguousarray(molwt, np.float64)
+    f = {k: _f32(grid[k]) for k in ("cgrid", "ta", "dens", "pres", "qv", "rj")}
+    f32p = C.POINTER(C.c_float)
+    zone = _f32(grid["seawater_zone"]) if grid.get("seawater_zone") is not None else None
+    ice = _f32(grid["seaice"]) if grid.get("seaice") is not None else None
+    kh = grid.get("khetero")
+    kh = np.ascontiguousarray(kh, np.float64) if kh is not None else None
+    ld = kh.shape[1] if kh is not None else max(nhet, 1)
+    out = {k: np.zeros(n) for k in ("temp", "pres_atm", "h2o", "sea")}
+    out["rj"] = np.zeros((nph, n))
+    out["het"] = np.zeros((max(nhet, 1), n))
+    out["y"] = np.zeros((ns, n))
+
+    def fp(a):
+        return None if a is None else _p(a, f32p)
+
+    lib().om_gather(ns, nph, nhet, nc, nr_, nl, _p(idx, _i32p), _p(conv, _i32p), _p(mw, _f64p),
+                    fp(f["cgrid"]), fp(f["ta"]), fp(f["dens"]), fp(f["pres"]), fp(f["qv"]), fp(zone), fp(ice),
+                    fp(f["rj"]), None if kh is None else _p(kh, _f64p), ld,
+                    _p(out["temp"], _f64p), _p(out["pres_atm"], _f64p), _p(out["h2o"], _f64p),
+                    _p(out["sea"], _f64p), _p(out["rj"], _f64p), _p(out["het"], _f64p), _p(out["y"], _f64p))
+    return out
+
+
+def chem_scatter(mech, grid, cgrid_index, convert_conc, molwt, y):
+    """CGRID (a float32 copy of grid["cgrid"]) with the mechanism species
+    overwritten from y [ns][n] (mechanism order)."""
+    nc, nr_, nl = grid["ncols"], grid["nrows"], grid["nlays"]
+    cg = np.array(grid["cgrid"], np.float32, copy=True, order="C")
+    idx = np.ascontiguousarray(cgrid_index, np.int32)
+    conv = np.ascontiguousarray(convert_conc, np.int32)
+    mw = np.ascontiguousarray(molwt, np.float64)
+    dens = _f32(grid["dens"])
+    yy = np.ascontiguousarray(y, np.float64)
+    f32p = C.POINTER(C.c_float)
+    lib().om_scatter(mech["numb_mech_spc"], nc, nr_, nl, _p(idx, _i32p), _p(conv, _i32p), _p(mw, _f64p),
+                     _p(dens, f32p), _p(yy, _f64p), _p(cg, f32p))
+    return cg

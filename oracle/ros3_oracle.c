@@ -1180,3 +1180,80 @@ int om_ddm_cell(const om_ctx *c, double chemstep, const double *rki, double *y,
     free(yave); free(jac); free(a); free(a1); free(ipvt); free(sold); free(prd); free(prate_v);
     return 0;
 }
+
+/* ---------------------------------------------------------------------------
+ * CHEM's CGRID gather and scatter (rbdriver.F:460-509, 577-595).  Every
+ * operation is REAL(4) as in the driver, REAL(8) where the driver widens.
+ * ------------------------------------------------------------------------- */
+static const float OM_CONCMIN = 1.0e-30f;                 /* rbdriver.F:110 */
+static const float OM_CONCOFM = 1.0e+06f;                 /* rbdriver.F:111 */
+static const float OM_STDATMPA = 101325.0f;               /* CONST.EXT:95   */
+static const float OM_MWAIR = 28.9628f, OM_MWWAT = 18.0153f;   /* CONST.EXT:113, 119 */
+
+static float om_forward_conv(double molwt)               /* rbinit.F:260   */
+{
+    float a = 1.0e-3f * OM_MWAIR;
+    return a / (float)molwt;
+}
+
+void om_gather(int ns, int nphotab, int nhetero, int ncols, int nrows, int nlays,
+               const int32_t *cgrid_index, const int32_t *convert_conc,
+               const double *species_molwt, const float *cgrid, const float *ta,
+               const float *dens, const float *pres, const float *qv,
+               const float *seawater_zone, const float *seaice, const float *rj,
+               const double *khetero, int ld_het, double *temp, double *pres_atm,
+               double *h2o, double *sea, double *rjd, double *het, double *y)
+{
+    const float pa2atm = 1.0f / OM_STDATMPA;             /* rbdriver.F:112 */
+    const float maomv = OM_MWAIR / OM_MWWAT;              /* rbdriver.F:113 */
+    const float qv_to_ppm = OM_CONCOFM * maomv;           /* rbdriver.F:114 */
+    const size_t n = (size_t)ncols * nrows * nlays;
+    float *svol = (float *)malloc(n * sizeof(float));
+    for (int lev = 0; lev < nlays; ++lev)
+        for (int row = 0; row < nrows; ++row)
+            for (int col = 0; col < ncols; ++col) {
+                size_t c = (size_t)col + (size_t)ncols * (row + (size_t)nrows * lev);
+                size_t c2 = (size_t)col + (size_t)ncols * row;
+                float zone = seawater_zone ? seawater_zone[c2] : 0.0f;
+                float ice = seaice ? seaice[c2] : 0.0f;
+                float s, h;
+                temp[c] = (double)ta[c];                                 /* :465 */
+                svol[c] = 1.0f / dens[c];                                /* :467 */
+                pres_atm[c] = (double)(pa2atm * pres[c]);                /* :468 */
+                h = qv_to_ppm * qv[c];                                   /* :469 */
+                h2o[c] = (double)(h > 0.0f ? h : 0.0f);
+                s = zone - ice;                                          /* :470 */
+                sea[c] = (double)(0.0f > s ? 0.0f : s);
+            }
+    for (int isp = 0; isp < ns; ++isp) {                                 /* :476-492 */
+        const size_t spc = (size_t)(cgrid_index[isp] - 1);
+        const float fwd = om_forward_conv(species_molwt[isp]);
+        for (size_t c = 0; c < n; ++c) {
+            float v = cgrid[spc * n + c];
+            if (convert_conc[isp]) v = fwd * svol[c] * v;
+            y[(size_t)isp * n + c] = (double)(v > OM_CONCMIN ? v : OM_CONCMIN);
+        }
+    }
+    for (int k = 0; k < nphotab; ++k)                                     /* :507-508 */
+        for (size_t c = 0; c < n; ++c) rjd[(size_t)k * n + c] = (double)rj[(size_t)k * n + c];
+    if (het)
+        for (int k = 0; k < nhetero; ++k)                                 /* :504-505 */
+            for (size_t c = 0; c < n; ++c)
+                het[(size_t)k * n + c] = khetero ? khetero[c * (size_t)ld_het + k] : 0.0;
+    free(svol);
+}
+
+void om_scatter(int ns, int ncols, int nrows, int nlays, const int32_t *cgrid_index,
+                const int32_t *convert_conc, const double *species_molwt, const float *dens,
+                const double *y, float *cgrid)
+{
+    const size_t n = (size_t)ncols * nrows * nlays;
+    for (int isp = 0; isp < ns; ++isp) {                                 /* :580-595 */
+        const size_t spc = (size_t)(cgrid_index[isp] - 1);
+        const double rev = (double)(1.0f / om_forward_conv(species_molwt[isp]));   /* rbinit.F:261 */
+        for (size_t c = 0; c < n; ++c) {
+            const double yc = y[(size_t)isp * n + c];
+            cgrid[spc * n + c] = convert_conc[isp] ? (float)(rev * (double)dens[c] * yc) : (float)yc;
+        }
+    }
+}

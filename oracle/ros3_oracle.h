@@ -125,6 +125,23 @@ int om_adjoint_run(const om_ctx *c, int64_t ncells, double chemstep, const doubl
 int om_ddm_cell(const om_ctx *c, double chemstep, const double *rki, double *y,
                 const double *rtol, const double *atol, int np, const int32_t *prate, float *sens);
 
+/* CHEM's CGRID gather (rbdriver.F:460-509) and scatter (rbdriver.F:577-595)
+ * in the driver's REAL(4) arithmetic, for ncells = ncols*nrows*nlays cells in
+ * CGRID order (Fortran column-major arrays, CONC(col,row,lay,spc) at
+ * cgrid[spc*ncells + cell]).  Per mechanism species (old order): cgrid_index
+ * (1-based), convert_conc, species_molwt.  Outputs are cell fastest; y in
+ * mechanism (old) order.  seawater_zone / seaice / khetero may be NULL. */
+void om_gather(int ns, int nphotab, int nhetero, int ncols, int nrows, int nlays,
+               const int32_t *cgrid_index, const int32_t *convert_conc,
+               const double *species_molwt, const float *cgrid, const float *ta,
+               const float *dens, const float *pres, const float *qv,
+               const float *seawater_zone, const float *seaice, const float *rj,
+               const double *khetero, int ld_het, double *temp, double *pres_atm,
+               double *h2o, double *sea, double *rjd, double *het, double *y);
+void om_scatter(int ns, int ncols, int nrows, int nlays, const int32_t *cgrid_index,
+                const int32_t *convert_conc, const double *species_molwt, const float *dens,
+                const double *y, float *cgrid);
+
 #ifdef __cplusplus
 }
 #endif
