@@ -273,7 +273,7 @@ def chem_io_leg(args, mech, solver, d, dims, dev):
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     ev[0].record()
     for _ in range(reps):
-        solver.gather_device(grid)
+        solver.gather_device(grid, out=blk)
     ev[1].record()
     for _ in range(reps):
         solver.scatter_device(grid, blk["y"])

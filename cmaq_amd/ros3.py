@@ -354,15 +354,20 @@ class Ros3Solver:
             raise self._err(rc, fail)
         return fail if rc != 0 else None
 
-    def gather_device(self, grid, stream=None):
-        """The block arrays of every cell (dict of float64 CUDA tensors)."""
+    def gather_device(self, grid, out=None, stream=None):
+        """The block arrays of every cell (dict of float64 CUDA tensors);
+        `out` (a dict returned by an earlier call) is reused when given."""
         import torch
         st = self._grid(grid, device=True)
         n = st.ncols * st.nrows * st.nlays
         dev = grid["ta"].device
-        z = lambda r: torch.zeros((r, n), dtype=torch.float64, device=dev)
-        out = {"temp": z(1)[0], "pres_atm": z(1)[0], "h2o": z(1)[0], "sea": z(1)[0], "rj": z(self.nphotab),
-               "het": z(self.nhetero) if grid.get("khetero") is not None else None, "y": z(self.ns)}
+        if out is None:
+            z = lambda r: torch.empty((r, n), dtype=torch.float64, device=dev)
+            out = {"temp": z(1)[0], "pres_atm": z(1)[0], "h2o": z(1)[0], "sea": z(1)[0], "rj": z(self.nphotab),
+                   "het": z(self.nhetero) if grid.get("khetero") is not None else None, "y": z(self.ns)}
+        self._check([("temp", out["temp"], 1), ("pres_atm", out["pres_atm"], 1), ("h2o", out["h2o"], 1),
+                     ("sea", out["sea"], 1), ("rj", out["rj"], self.nphotab), ("het", out["het"], self.nhetero),
+                     ("y", out["y"], self.ns)], n)
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         rc = lib().cmaq_ros3_gather_device(self._ctx, C.byref(st), _ptr(out["temp"]), _ptr(out["pres_atm"]),
