@@ -226,6 +226,24 @@ def adjoint_leg(args, mech, solver, dev):
     rep, rev, chunks = solver.last_adjoint_timing()
     print(f"[bench] adjoint leg: forward+adjoint {t_both * 1e3:.0f} ms", file=sys.stderr, flush=True)
     bad = int((~torch.isfinite(gy)).any(dim=0).sum())
+    # one sync step with one tangent-linear direction (d ln O3 of every cell
+    # and d ln k of all reactions at once) against the plain solve
+    y.copy_(d["y"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.solve_device(5.0, rki, y, na, raise_on_fail=False)
+    torch.cuda.synchronize()
+    t_one = time.perf_counter() - t0
+    y.copy_(d["y"])
+    dy = torch.zeros_like(y)
+    dy[io3] = d["y"][io3]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.tlm_device(5.0, rki, y, dy, rki, nacc=na, raise_on_fail=False)
+    torch.cuda.synchronize()
+    t_tlm = time.perf_counter() - t0
+    print(f"[bench] tangent-linear: one sync step {t_one * 1e3:.0f} ms, with one direction {t_tlm * 1e3:.0f} ms",
+          file=sys.stderr, flush=True)
     return t_both / t_fwd, {
         "workload": f"{mech['mechname'].lower()} ROS3, mixed regimes{' + VOC/SVOC' if args.adj_svoc else ''}, "
                     f"{n} cells, {k} sync steps of 5 min; "
@@ -234,7 +252,9 @@ def adjoint_leg(args, mech, solver, dev):
         "ncells": n, "sync_steps": k, "variant": solver.variant, "fwd_ms": t_fwd * 1e3, "fwd_adj_ms": t_both * 1e3,
         "accepted_cell_steps": accepted,
         "last_sync_step_replay_ms": rep, "last_sync_step_reverse_ms": rev, "tape_chunks": chunks,
-        "cells_without_gradient": bad}
+        "cells_without_gradient": bad,
+        "tlm_one_direction_over_fwd": t_tlm / t_one, "tlm_sync_step_ms": t_tlm * 1e3,
+        "fwd_sync_step_ms": t_one * 1e3}
 
 
 def chem_io_leg(args, mech, solver, d, dims, dev):

@@ -38,7 +38,7 @@ VARIANTS = {
     "cracmm1_aq": [(110, 8, "tile12x12+k1")],
     "cracmm1amore_aq": [(110, 8, "tile12x12+k1")],
     "saprc07tc_ae6_aq": [(110, 8, "tile12x12+k1")],
-    "saprc07tic_ae7i_aq": [(110, 8, "tile12x12+k1")],
+    "saprc07tic_ae7i_aq": [(110, 8, "tile20x20+k1")],
 }
 
 

@@ -28,7 +28,8 @@ import re
 import struct
 
 from .programs import (adj_a_ops, adj_b_ops, adj_seed_ops, adj_u1_ops, adj_u2_ops, feval_ops,
-                       jac_lu_ops, solve_ops, tsolve_ops, yp_ops)
+                       jac_lu_ops, msolve_ops, solve_ops, tlm_rhs1_ops, tlm_rhs_ops, tlm_stage_ops,
+                       tlm_update_ops, tsolve_ops, yp_ops)
 from .regalloc import Op, allocate
 from .symbolic import analyse
 
@@ -232,11 +233,30 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
             ("adj_u2", adj_u2_ops(sym), {"U2"}, ", double x32", None),
             ("adj_a", adj_a_ops(sym), {"YB", "S2", "S3"}, "", None),
             ("adj_u1", adj_u1_ops(sym), {"U1"}, ", double x21, double x31", None),
-            ("adj_b", adj_b_ops(sym), {"LAM", "GK"}, "", None)):
+            ("adj_b", adj_b_ops(sym), {"LAM", "GK"}, "", None),
+            # tangent-linear direction through one accepted step
+            ("tlm_rhs1", tlm_rhs1_ops(sym), {"U1"}, "", None),
+            ("msolve", msolve_ops(sym, "X"), {"X"}, ", int x_off_", {"X": "x_off_"}),
+            ("tlm_stage", tlm_stage_ops(sym), {"S2"}, "", None),
+            ("tlm_rhs2", tlm_rhs_ops(sym, 2), {"U2"}, ", double x1", None),
+            ("tlm_rhs3", tlm_rhs_ops(sym, 3), {"U3"}, ", double x1, double x2", None),
+            ("tlm_update", tlm_update_ops(sym), {"LAM"}, "", None)):
         items, cnt = allocate(ops, outs, budget, lookahead, entry_all)
         stats[fname] = _stat(ops, cnt)
         body += fn(fname, params, items, off, budget, dyn)
         b("")
+    b("__device__ __noinline__ void tlm_step(double *base, double dtinv)")
+    b("{")
+    b("    tlm_rhs1(base);")
+    b("    msolve(base, OFF_U1);")
+    b("    tlm_stage(base);")
+    b("    tlm_rhs2(base, C21 * dtinv);")
+    b("    msolve(base, OFF_U2);")
+    b("    tlm_rhs3(base, C31 * dtinv, C32 * dtinv);")
+    b("    msolve(base, OFF_U3);")
+    b("    tlm_update(base);")
+    b("}")
+    b("")
 
     out = []
     w = out.append
@@ -303,6 +323,21 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w("    if (P.grad_rki)")
     w("        for (int r = 0; r < NR; ++r) P.grad_rki[r * n + cell] += LD(OFF_GK + r);")
     w("}")
+    w("__device__ __noinline__ void load_tlm_cell(const SolveParams &P, int64_t cell, double *base_)")
+    w("{")
+    w("    ROS3_PROLOGUE")
+    w("    const int64_t n = P.stride;")
+    w("    for (int r = 0; r < NR; ++r) ST(OFF_GK + r, P.dk ? P.dk[r * n + cell] : 0.0);")
+    for i in range(ns):
+        w(f"    ST(OFF_LAM + {i}, P.lam[{sym.inew2old[i]} * n + cell]);")
+    w("}")
+    w("__device__ __noinline__ void store_tlm_cell(const SolveParams &P, int64_t cell, double *base_, bool failed)")
+    w("{")
+    w("    ROS3_PROLOGUE")
+    w("    const int64_t n = P.stride;")
+    for i in range(ns):
+        w(f"    P.lam[{sym.inew2old[i]} * n + cell] = failed ? __builtin_nan(\"\") : LD(OFF_LAM + {i});")
+    w("}")
     w("__device__ __noinline__ void tape_store(double *base_, double *rec, double h)")
     w("{")
     w("    ROS3_PROLOGUE")
@@ -337,5 +372,5 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w(f"    (const void *)&ros3::{cname}::solve_kernel_t<false>, ros3::{cname}::INEW2OLD_HOST,")
     w(f"    \"{variant}\", {budget}, {lookahead}, ros3::{cname}::WAVES_PER_EU, {1 if is_default else 0}, {refill_min},")
     w(f"    (const void *)&ros3::{cname}::solve_kernel_t<true>, (const void *)&ros3::{cname}::reverse_kernel,")
-    w(f"    ros3::{cname}::TAPE_RS}};")
+    w(f"    ros3::{cname}::TAPE_RS, (const void *)&ros3::{cname}::solve_kernel_t<false, true>}};")
     return "\n".join(out) + "\n", sym

@@ -463,6 +463,137 @@ def op_flops(ops):
     return n
 
 
+# ---------------------------------------------------------------------------
+# Tangent-linear (forward-mode) sensitivity of one accepted ROS3 step along a
+# direction (dY, dk), step size held fixed (no reference counterpart):
+#   U1 = -S w1,  w1_r = q_r(Y; dY) + dk_r p_r(Y) + dq_r(Y; K1);  U1 <- M^-1 U1
+#   S3 = Y + K1 (stage point), YB = dY + U1,  S2 = S (q_r(S3; YB) + dk_r p_r(S3))
+#   U2 = M^-1 (-S2 - x21 U1 - S dq_r(Y; K2))
+#   U3 = M^-1 (-S2 - x31 U1 - x32 U2 - S dq_r(Y; K3))
+#   dY <- [Ynew > CONMIN] (dY + B1 U1 + B2 U2 + B3 U3)
+# with p_r the reactant product, q_r(y; v) = k_r grad p_r(y) . v and
+# dq_r(Y; K) = dk_r grad p_r(Y) . K + k_r (d/de grad p_r(Y + e dY)) . K.
+# dY lives in LAM and dk in GK (the adjoint's arrays, unused by the forward);
+# the first ns entries of S2 / S3 hold dF1 and the stage point.
+# ---------------------------------------------------------------------------
+
+def _scatter(sym, r, out, sign):
+    """out += sign * (S w)_r with w in RATE(r): -w per reactant, +sc w per product."""
+    ops = []
+    for a in sym.reactants[r]:
+        ops.append(Op("{0} = {0} %s {1};" % ("-" if sign > 0 else "+"), [(out, a), ("RATE", r)], True))
+    for k, p in enumerate(sym.products[r]):
+        c = sym.sc[r][k]
+        op = "+" if sign > 0 else "-"
+        if c == 1.0:
+            ops.append(Op("{0} = {0} %s {1};" % op, [(out, p), ("RATE", r)], True))
+        else:
+            ops.append(Op("{0} = {0} %s %s * {1};" % (op, _lit(c)), [(out, p), ("RATE", r)], True))
+    return ops
+
+
+def _tlm_w(sym, r, terms):
+    """RATE(r) = sum of the requested per-reaction terms:
+    'q'  q_r(YV; DV)     'p'  dk_r p_r(YV)     'dq' dq_r(Y; KV) along (LAM, GK)
+    terms: list of (kind, {name: array})."""
+    re = sym.reactants[r]
+    args = [("RATE", r)]
+    index = {}
+
+    def slot(arr, i):
+        key = (arr, i)
+        if key not in index:
+            args.append(key)
+            index[key] = len(args) - 1
+        return "{%d}" % index[key]
+
+    k, g = (lambda: slot("RK", r)), (lambda: slot("GK", r))
+    parts = []
+    for kind, arrs in terms:
+        if kind == "q":
+            y, d = arrs["YV"], arrs["DV"]
+            if len(re) == 1:
+                parts.append("%s * %s" % (k(), slot(d, re[0])))
+            else:
+                a, b = re
+                parts.append("%s * (%s * %s + %s * %s)" % (k(), slot(d, a), slot(y, b), slot(d, b), slot(y, a)))
+        elif kind == "p":
+            y = arrs["YV"]
+            if len(re) == 1:
+                parts.append("%s * %s" % (g(), slot(y, re[0])))
+            else:
+                parts.append("%s * (%s * %s)" % (g(), slot(y, re[0]), slot(y, re[1])))
+        else:  # dq along (LAM, GK) at Y
+            kv = arrs["KV"]
+            if len(re) == 1:
+                parts.append("%s * %s" % (g(), slot(kv, re[0])))
+            else:
+                a, b = re
+                parts.append("%s * (%s * %s + %s * %s)" % (g(), slot(kv, a), slot("Y", b), slot(kv, b), slot("Y", a)))
+                parts.append("%s * (%s * %s + %s * %s)" % (k(), slot(kv, a), slot("LAM", b), slot(kv, b),
+                                                          slot("LAM", a)))
+    return Op("{0} = " + " + ".join(parts) + ";", args, False)
+
+
+def tlm_rhs1_ops(sym):
+    """U1 = -S (q(Y; dY) + dk p(Y) + dq(Y; K1)), before the solve."""
+    _check_bimolecular(sym)
+    ops = [Op("{0} = 0.0;", [("U1", i)], False) for i in range(sym.ns)]
+    for r in sym.day.active:
+        ops.append(_tlm_w(sym, r, [("q", {"YV": "Y", "DV": "LAM"}), ("p", {"YV": "Y"}), ("dq", {"KV": "K1"})]))
+        ops += _scatter(sym, r, "U1", -1)
+    return ops
+
+
+def tlm_stage_ops(sym):
+    """S3 = Y + K1 (the stage point, as yp_ops), YB = dY + U1, then
+    S2 = S (q(S3; YB) + dk p(S3))."""
+    ops = [Op("{0} = {1} + {2};", [("S3", i), ("Y", i), ("K1", i)], False) for i in range(sym.ns)]
+    ops += [Op("{0} = {1} + {2};", [("YB", i), ("LAM", i), ("U1", i)], False) for i in range(sym.ns)]
+    ops += [Op("{0} = 0.0;", [("S2", i)], False) for i in range(sym.ns)]
+    for r in sym.day.active:
+        ops.append(_tlm_w(sym, r, [("q", {"YV": "S3", "DV": "YB"}), ("p", {"YV": "S3"})]))
+        ops += _scatter(sym, r, "S2", 1)
+    return ops
+
+
+def tlm_rhs_ops(sym, stage):
+    """U2 = -S2 - x1 U1 - S dq(Y; K2), or U3 = -S2 - x1 U1 - x2 U2 - S dq(Y; K3)."""
+    _check_bimolecular(sym)
+    out, kv = ("U2", "K2") if stage == 2 else ("U3", "K3")
+    ops = []
+    for i in range(sym.ns):
+        if stage == 2:
+            ops.append(Op("{0} = -{1} - x1 * {2};", [(out, i), ("S2", i), ("U1", i)], False))
+        else:
+            ops.append(Op("{0} = -{1} - x1 * {2} - x2 * {3};", [(out, i), ("S2", i), ("U1", i), ("U2", i)], False))
+    for r in sym.day.active:
+        ops.append(_tlm_w(sym, r, [("dq", {"KV": kv})]))
+        ops += _scatter(sym, r, out, -1)
+    return ops
+
+
+def msolve_ops(sym, x):
+    """x <- (L U)^-1 x with the factors of jac_lu (RBSOLVE's substitutions,
+    rbsolve.F:78-187, on an arbitrary right-hand side)."""
+    day = sym.day
+    ops = []
+    for i in range(sym.ns):
+        for (s, c) in day.lower[i]:
+            ops.append(Op("{0} = {0} - {1} * {2};", [(x, i), ("LU", s), (x, c)], True))
+    for i in range(sym.ns - 1, -1, -1):
+        for (s, c) in day.upper[i]:
+            ops.append(Op("{0} = {0} - {1} * {2};", [(x, i), ("LU", s), (x, c)], True))
+        ops.append(Op("{0} = {0} * {1};", [(x, i), ("VD", i)], True))
+    return ops
+
+
+def tlm_update_ops(sym):
+    """dY <- [Ynew > CONMIN] (dY + B1 U1 + B2 U2 + B3 U3); Ynew is in YP."""
+    return [Op("{0} = {1} > CONMIN ? (({0} + B1 * {2}) + B2 * {3}) + B3 * {4} : 0.0;",
+               [("LAM", i), ("YP", i), ("U1", i), ("U2", i), ("U3", i)], True) for i in range(sym.ns)]
+
+
 def flops_per_step(sym):
     """FP64 operations of one attempted step: two f evaluations, Jacobian and
     LU, three stage solves, YP = Y + K1, and the new state and error norm of

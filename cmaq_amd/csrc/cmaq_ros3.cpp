@@ -46,6 +46,7 @@ struct cmaq_ros3_ctx {
     int waves_per_cu = 0;
     int grid_blocks = 0;      // solver workgroups
     int grid_replay = 0, grid_reverse = 0;   // adjoint kernels' workgroups
+    int grid_tlm = 0;                          // tangent-linear kernel's workgroups
     int num_cus = 0;
     int64_t waves = 0;
     // device buffers
@@ -341,12 +342,13 @@ static int grid_for(cmaq_ros3_ctx *c, const void *kernel, int *grid)
 
 static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
 {
-    int grid = 0, grid_a = 0, grid_b = 0;
+    int grid = 0, grid_a = 0, grid_b = 0, grid_t = 0;
     if (int rc = grid_for(c, mk->solve_kernel, &grid)) return rc;
     if (int rc = grid_for(c, mk->replay_kernel, &grid_a)) return rc;
     if (int rc = grid_for(c, mk->reverse_kernel, &grid_b)) return rc;
+    if (int rc = grid_for(c, mk->tlm_kernel, &grid_t)) return rc;
     const int wpb = kSolveBlock / kWave;
-    const int64_t waves = (int64_t)std::max(grid, std::max(grid_a, grid_b)) * wpb;
+    const int64_t waves = (int64_t)std::max(std::max(grid, grid_t), std::max(grid_a, grid_b)) * wpb;
     // per-wave workspace stride: NWS rows of 64 doubles plus an optional pad
     // of rows that staggers the waves' bases across memory channels
     int64_t pad = 0;
@@ -365,6 +367,7 @@ static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
     c->grid_blocks = grid;
     c->grid_replay = grid_a;
     c->grid_reverse = grid_b;
+    c->grid_tlm = grid_t;
     c->ws_stride = stride;
     c->waves = (int64_t)(c->ws_doubles / (size_t)stride);
     return 0;
@@ -616,14 +619,21 @@ static int check_fail(cmaq_ros3_ctx *c, const unsigned char *small, cmaq_ros3_fa
 // Solver kernel over n cells with rate constants rki already on the device;
 // ev_from is the event recorded before the rates (or before the solve).
 static int launch_solve(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki, double *y,
-                        int32_t *nacc, int32_t *nrej, cmaq_ros3_fail *fail, hipStream_t s)
+                        int32_t *nacc, int32_t *nrej, cmaq_ros3_fail *fail, hipStream_t s,
+                        double *dy = nullptr, const double *dk = nullptr)
 {
     HIP_TRY(c, hipEventRecord(c->ev[1], s));
     SolveParams P;
     solve_params(c, P, n, chemstep, rki, y, nacc, nrej);
     if (c->user_order && c->user_order_n == n) P.order = c->user_order;
     void *args[] = {&P};
-    HIP_TRY(c, hipLaunchKernel(c->mk->solve_kernel, dim3(c->grid_blocks), dim3(kSolveBlock), args, 0, s));
+    if (dy) {   // the forward with one tangent-linear direction
+        P.lam = dy;
+        P.dk = dk;
+        HIP_TRY(c, hipLaunchKernel(c->mk->tlm_kernel, dim3(c->grid_tlm), dim3(kSolveBlock), args, 0, s));
+    } else {
+        HIP_TRY(c, hipLaunchKernel(c->mk->solve_kernel, dim3(c->grid_blocks), dim3(kSolveBlock), args, 0, s));
+    }
     HIP_TRY(c, hipEventRecord(c->ev[2], s));
     unsigned char small[64];
     HIP_TRY(c, hipMemcpyAsync(small, c->d_small, 64, hipMemcpyDeviceToHost, s));
@@ -682,6 +692,20 @@ extern "C" int cmaq_ros3_solve_device(cmaq_ros3_ctx *c, int64_t n, double chemst
     HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
     HIP_TRY(c, hipEventRecord(c->ev[0], s));
     return launch_solve(c, n, chemstep, rki, y, nacc, nrej, fail, s);
+}
+
+extern "C" int cmaq_ros3_tlm_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,
+                                    double *y, int32_t *nacc, int32_t *nrej, double *dy, const double *dk,
+                                    cmaq_ros3_fail *fail, void *stream)
+{
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (n > 0 && (!rki || !y || !dy)) return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
+    HIP_TRY(c, hipEventRecord(c->ev[0], s));
+    return launch_solve(c, n, chemstep, rki, y, nacc, nrej, fail, s, dy, dk);
 }
 
 extern "C" int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,

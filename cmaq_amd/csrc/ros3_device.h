@@ -256,8 +256,10 @@ struct SolveParams {
     double *tape;             // record storage of this launch
     const int64_t *tape_off;  // [stride] first record of each cell
     const int32_t *nacc_fwd;  // [stride] accepted steps of the forward run
-    double *lam;              // [ns][stride] mechanism order: dL/dy_end in, dL/dy0 out
+    double *lam;              // [ns][stride] mechanism order: dL/dy_end in, dL/dy0 out;
+                              // tangent-linear run: dy0 in, dy_end out
     double *grad_rki;         // [nr][stride] accumulated dL/drki, or nullptr
+    const double *dk;         // tangent-linear run: [nr][stride] rate direction, or nullptr
 };
 
 __device__ inline int lane_id() { return __lane_id(); }

@@ -19,6 +19,7 @@ struct MechKernels {
     const void *replay_kernel;        // solve_kernel writing per-step tapes
     const void *reverse_kernel;       // discrete adjoint over the tapes
     int tape_rs;                      // doubles per tape record
+    const void *tlm_kernel;           // solve_kernel carrying a tangent-linear direction
 };
 
 }  // namespace ros3

@@ -104,6 +104,7 @@ def lib():
         L.cmaq_ros3_set_variant.argtypes = [vp, C.c_char_p]
         L.cmaq_ros3_variant.argtypes = [vp, C.c_int]
         L.cmaq_ros3_variant.restype = C.c_char_p
+        L.cmaq_ros3_tlm_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 6 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_set_cgrid_map.argtypes = [vp, C.c_int32, _i32p, _i32p, _f64p]
         L.cmaq_ros3_chem.argtypes = [vp, C.POINTER(_Grid), C.c_double, _i32p, _i32p, C.POINTER(_Fail)]
         L.cmaq_ros3_chem_device.argtypes = [vp, C.POINTER(_Grid), C.c_double, vp, vp, C.POINTER(_Fail), vp]
@@ -447,6 +448,23 @@ class Ros3Solver:
         f = _Fail()
         rc = lib().cmaq_ros3_solve_device(self._ctx, n, chemstep, _ptr(rki), _ptr(y), _ptr(nacc), _ptr(nrej),
                                           C.byref(f), C.c_void_p(stream))
+        fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, fail)
+        return fail if rc != 0 else None
+
+    def tlm_device(self, chemstep, rki, y, dy, dk=None, nacc=None, nrej=None, stream=None, raise_on_fail=True):
+        """solve_device plus one tangent-linear direction: y [ns, n] advances
+        in place, dy [ns, n] (dy0 in, d y_end out) along dk [nr, n] (optional)."""
+        import torch
+        n = y.numel() // self.ns
+        self._check([("rki", rki, self.nr), ("y", y, self.ns), ("dy", dy, self.ns), ("dk", dk, self.nr),
+                     ("nacc", nacc, 1), ("nrej", nrej, 1)], n)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_ros3_tlm_device(self._ctx, n, chemstep, _ptr(rki), _ptr(y), _ptr(nacc), _ptr(nrej),
+                                        _ptr(dy), _ptr(dk), C.byref(f), C.c_void_p(stream))
         fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
         if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
             raise self._err(rc, fail)

@@ -82,6 +82,7 @@ def lib():
         L.om_adjoint_run.argtypes = [vp, C.c_int64, C.c_double] + [_f64p] * 7 + [_i32p]
         L.om_ddm_cell.argtypes = [vp, C.c_double, _f64p, _f64p, _f64p, _f64p, C.c_int, _i32p,
                                   C.POINTER(C.c_float)]
+        L.om_tlm_cell.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, C.c_int, _f64p, _f64p]
         _f32p = C.POINTER(C.c_float)
         L.om_gather.argtypes = [C.c_int] * 6 + [_i32p, _i32p, _f64p] + [_f32p] * 8 + [_f64p, C.c_int] + \
             [_f64p] * 7
@@ -283,6 +284,17 @@ class Oracle:
         lib().om_adjoint_cell(self._ctx, 1, _p(rk, _f64p), _p(y0n, _f64p), _p(h, _f64p), h.size,
                               _p(ln, _f64p), _p(gk, _f64p))
         return self._to_old(ln), gk
+
+    def tlm_cell(self, rki, y0, h, dy0, dk=None):
+        """d y_end along (dy0, dk) for the fixed steps h (mechanism order)."""
+        rk = np.ascontiguousarray(rki, np.float64)
+        yn = self._to_new(y0)
+        dy = self._to_new(dy0)
+        hh = np.ascontiguousarray(h, np.float64)
+        dkp = None if dk is None else np.ascontiguousarray(dk, np.float64)
+        lib().om_tlm_cell(self._ctx, 1, _p(rk, _f64p), _p(yn, _f64p), _p(hh, _f64p), len(hh), _p(dy, _f64p),
+                          None if dkp is None else _p(dkp, _f64p))
+        return self._to_old(dy)
 
     def adjoint_run(self, chemstep, rki, y0, lam, rtol=1e-3, atol=1e-7):
         """All cells: returns dict(lam=dL/dy0, gk=dL/drki, y=y_end, nacc)."""

@@ -989,6 +989,108 @@ void om_adjoint_cell(const om_ctx *c, int ncsp, const double *rki, const double 
     free(y); free(tape);
 }
 
+/* ------------------------------------------------------------------ */
+/* Tangent-linear (forward-mode) sensitivity of the same fixed-step map
+ * (no reference counterpart; an exact alternative to DDM-3D's CN step,
+ * ddm3d/DDM3D_CHEM.F:155-511).  Along a direction (dy0, dk), one step with
+ * M = J(y) - I/(GAM h):
+ *   dK1 = M^-1 (-S w1),  w1_r = q_r(y; dy) + dk_r p_r(y) + dq_r(y; K1)
+ *   dYP = dy + dK1,      dF1 = S (q_r(yp; dYP) + dk_r p_r(yp))
+ *   dK2 = M^-1 (-dF1 - x21 dK1 - S dq_r(y; K2))
+ *   dK3 = M^-1 (-dF1 - x31 dK1 - x32 dK2 - S dq_r(y; K3))
+ *   dy_new = [yn > CONMIN] (dy + B1 dK1 + B2 dK2 + B3 dK3)
+ * with p_r the reactant product, q_r(y; v) = k_r grad p_r(y) . v its
+ * directional derivative and dq_r(y; K) = d/de [k_r grad p_r(y) . K] along
+ * (dy, dk): the Jacobian's change applied to a stage. */
+
+/* q_r(y; v) / k_r */
+static double om_qdir(const om_ctx *c, int r, const double *y, const double *v)
+{
+    double s = 0.0;
+    for (int q = 1; q <= c->t.nreact[r - 1]; ++q) s += v[IRM2(c, r, q) - 1] * pexcept(c, r, y, q, 0);
+    return s;
+}
+
+/* dq_r(y; K) along (dy, dk_r) */
+static double om_dq(const om_ctx *c, int r, double k, double dkr, const double *y, const double *dy,
+                    const double *K)
+{
+    const int nre = c->t.nreact[r - 1];
+    double s = dkr * om_qdir(c, r, y, K);
+    for (int q = 1; q <= nre; ++q)
+        for (int q2 = 1; q2 <= nre; ++q2)
+            if (q2 != q) s += k * K[IRM2(c, r, q) - 1] * dy[IRM2(c, r, q2) - 1] * pexcept(c, r, y, q, q2);
+    return s;
+}
+
+/* out += sgn * S w (w per reaction, 1-based) */
+static void om_smul(const om_ctx *c, int ncsp, const double *w, double sgn, double *out)
+{
+    for (int nrx = 1; nrx <= c->nuserat[ncsp]; ++nrx) {
+        int r = c->nkuserat[ncsp][nrx];
+        for (int q = 1; q <= c->t.nreact[r - 1]; ++q) out[IRM2(c, r, q) - 1] -= sgn * w[r];
+        for (int p = 1; p <= c->t.nprdct[r - 1]; ++p) out[IRM2(c, r, p + 3) - 1] += sgn * SC(c, r, p) * w[r];
+    }
+}
+
+static void tlm_step1(const om_ctx *c, int ncsp, const double *rki, const double *y, double dt,
+                      double *dy, const double *dk, om_work *w, double *wr, double *df1)
+{
+    const int ns = c->ns;
+    ros3_stages1(c, ncsp, rki, y, dt, w->ydot, w->cc2, w->vdiag, w->k1, w->k2, w->k3, w->yp, w->yn);
+    const double dtinv = 1.0 / dt;
+    const double x21 = C21 * dtinv, x31 = C31 * dtinv, x32 = C32 * dtinv;
+    /* stage 1 */
+    for (int nrx = 1; nrx <= c->nuserat[ncsp]; ++nrx) {
+        int r = c->nkuserat[ncsp][nrx];
+        double k = rki[r - 1], dkr = dk ? dk[r - 1] : 0.0;
+        wr[r] = k * om_qdir(c, r, y, dy) + dkr * pexcept(c, r, y, 0, 0) + om_dq(c, r, k, dkr, y, dy, w->k1);
+    }
+    for (int i = 0; i < ns; ++i) w->u1[i] = 0.0;
+    om_smul(c, ncsp, wr, -1.0, w->u1);
+    om_solve(c, ncsp, 1, w->cc2, w->vdiag, w->u1);
+    /* dF1 at the stage point */
+    for (int i = 0; i < ns; ++i) w->yb[i] = dy[i] + w->u1[i];
+    for (int nrx = 1; nrx <= c->nuserat[ncsp]; ++nrx) {
+        int r = c->nkuserat[ncsp][nrx];
+        double k = rki[r - 1], dkr = dk ? dk[r - 1] : 0.0;
+        wr[r] = k * om_qdir(c, r, w->yp, w->yb) + dkr * pexcept(c, r, w->yp, 0, 0);
+    }
+    for (int i = 0; i < ns; ++i) df1[i] = 0.0;
+    om_smul(c, ncsp, wr, 1.0, df1);
+    /* stages 2 and 3 */
+    const double *K[2] = {w->k2, w->k3};
+    double *U[2] = {w->u2, w->u3};
+    for (int s = 0; s < 2; ++s) {
+        for (int nrx = 1; nrx <= c->nuserat[ncsp]; ++nrx) {
+            int r = c->nkuserat[ncsp][nrx];
+            wr[r] = om_dq(c, r, rki[r - 1], dk ? dk[r - 1] : 0.0, y, dy, K[s]);
+        }
+        for (int i = 0; i < ns; ++i)
+            U[s][i] = s == 0 ? -df1[i] - x21 * w->u1[i] : -df1[i] - x31 * w->u1[i] - x32 * w->u2[i];
+        om_smul(c, ncsp, wr, -1.0, U[s]);
+        om_solve(c, ncsp, 1, w->cc2, w->vdiag, U[s]);
+    }
+    for (int i = 0; i < ns; ++i)
+        dy[i] = w->yn[i] > CONMIN ? dy[i] + B1 * w->u1[i] + B2 * w->u2[i] + B3 * w->u3[i] : 0.0;
+}
+
+void om_tlm_cell(const om_ctx *c, int ncsp, const double *rki, const double *y0, const double *h, int nh,
+                 double *dy, const double *dk)
+{
+    const int ns = c->ns;
+    om_work w;
+    work_alloc(c, ncsp, &w);
+    double *y = xcalloc(ns, 8), *wr = xcalloc((size_t)c->nr + 1, 8), *df1 = xcalloc(ns, 8);
+    memcpy(y, y0, (size_t)ns * 8);
+    for (int n = 0; n < nh; ++n) {
+        tlm_step1(c, ncsp, rki, y, h[n], dy, dk, &w, wr, df1);
+        for (int q = 0; q < ns; ++q) y[q] = w.yn[q] > CONMIN ? w.yn[q] : CONMIN;
+    }
+    work_free(&w);
+    free(y); free(wr); free(df1);
+}
+
 int om_adjoint_run(const om_ctx *c, int64_t ncells, double chemstep, const double *rki,
                    const double *y0, const double *rtol, const double *atol,
                    double *lam, double *gk, double *yend, int32_t *nacc)

@@ -116,6 +116,12 @@ int om_adjoint_run(const om_ctx *c, int64_t ncells, double chemstep, const doubl
                    const double *y0, const double *rtol, const double *atol,
                    double *lam, double *gk, double *yend, int32_t *nacc);
 
+/* Tangent-linear sensitivity of the fixed-step map (steps h[0..nh-1]) along
+ * (dy, dk): dy [ns] (NEW order) in/out becomes d y_end; dk [nr] or NULL.
+ * No reference counterpart. */
+void om_tlm_cell(const om_ctx *c, int ncsp, const double *rki, const double *y0, const double *h, int nh,
+                 double *dy, const double *dk);
+
 /* CMAQ-DDM-3D chemistry step (ddm3d/DDM3D_CHEM.F:155-511) for one cell and
  * one sync step: advances y (NEW order) with per-cell RBSOLVER while
  * accumulating YAVE, then applies the REAL(4) Crank-Nicolson sensitivity

@@ -51,6 +51,11 @@ class Programs:
         self.a = compile_ops(P.adj_a_ops(sym))
         self.u1 = compile_ops(P.adj_u1_ops(sym), ["x21", "x31"])
         self.b = compile_ops(P.adj_b_ops(sym))
+        self.t_rhs1 = compile_ops(P.tlm_rhs1_ops(sym))
+        self.t_stage = compile_ops(P.tlm_stage_ops(sym))
+        self.t_rhs = {s: compile_ops(P.tlm_rhs_ops(sym, s), ["x1", "x2"]) for s in (2, 3)}
+        self.ms = {x: compile_ops(P.msolve_ops(sym, x)) for x in ("U1", "U2", "U3")}
+        self.t_update = compile_ops(P.tlm_update_ops(sym))
 
     def workspace(self, rk, y):
         ns, nr, nnz = self.sym.ns, self.sym.nr, self.sym.day.nnz
@@ -93,3 +98,20 @@ class Programs:
         self.u1(W, CONST["C21"] * dtinv, CONST["C31"] * dtinv)
         self.ts["U1"](W)
         self.b(W)
+
+    def tlm_step(self, W, h):
+        """W holds Y (step start), LAM = dY, GK = dk; on exit LAM = dYnew
+        (Y itself is not advanced)."""
+        self.stages(W, h)
+        dtinv = 1.0 / h
+        c = CONST
+        W["YP"] = [((y + c["B1"] * a) + c["B2"] * b) + c["B3"] * d
+                   for y, a, b, d in zip(W["Y"], W["K1"], W["K2"], W["K3"])]
+        self.t_rhs1(W)
+        self.ms["U1"](W)
+        self.t_stage(W)
+        self.t_rhs[2](W, c["C21"] * dtinv, 0.0)
+        self.ms["U2"](W)
+        self.t_rhs[3](W, c["C31"] * dtinv, c["C32"] * dtinv)
+        self.ms["U3"](W)
+        self.t_update(W)

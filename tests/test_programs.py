@@ -66,3 +66,25 @@ def test_adjoint_programs_match_oracle(cb6r5, oracle_cb6r5, prog_cb6r5):
         gk = np.array(W["GK"])
         np.testing.assert_allclose(lam, lam_ref, rtol=1e-10, atol=1e-12 * np.abs(lam_ref).max())
         np.testing.assert_allclose(gk, gk_ref, rtol=1e-10, atol=1e-12 * np.abs(gk_ref).max())
+
+
+def test_tlm_programs_match_oracle(cb6r5, oracle_cb6r5, prog_cb6r5):
+    """The generated tangent-linear programs against the oracle's
+    om_tlm_cell (different summation order: to rounding)."""
+    o, pg = oracle_cb6r5, prog_cb6r5
+    inew = o.inew2old() - 1
+    rng = np.random.default_rng(4)
+    for seed in (2, 5):
+        rki, y0 = _cell(cb6r5, o, seed)
+        h, _ = o.steps_cell(5.0, rki, y0)
+        dy0, dk = y0 * rng.normal(size=y0.size), rki * rng.normal(size=rki.size)
+        ref = o.tlm_cell(rki, y0, h, dy0, dk)
+        W = pg.workspace(rki, y0[inew])
+        W["LAM"] = list(dy0[inew])
+        W["GK"] = list(dk)
+        for hn in h:
+            pg.tlm_step(W, hn)
+            W["Y"] = pg.ynew(W)
+        got = np.empty(len(y0))
+        got[inew] = W["LAM"]
+        np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
