@@ -8,7 +8,10 @@ respect to them can be taken).  Its backward pass is the exact discrete
 adjoint of the integrator's accepted steps (step sizes held fixed), computed
 on the GPU by cmaq_ros3_adjoint_device: gradients with respect to the
 initial concentrations and to the rate constants (for log rate constants,
-take ``rki = logk.exp()``).
+take ``rki = logk.exp()``).  Forward-mode AD (``torch.autograd.forward_ad``,
+``torch.func.jvp``) is the tangent-linear counterpart: each sync step is
+rerun from its stored state by cmaq_ros3_tlm_device, which takes the same
+steps and carries the tangent through them.
 
 Memory: the forward keeps the state at the start of every sync step and the
 per-cell accepted-step counts; stage vectors are recomputed within each sync
@@ -39,7 +42,18 @@ class Ros3Integrate(torch.autograd.Function):
         ctx.solver, ctx.chemstep = solver, chemstep
         ctx.save_for_backward(rk, *states, *counts)
         ctx.nsteps = nsteps
+        ctx.fwd_states = (rk, states)       # for jvp
         return y
+
+    @staticmethod
+    def jvp(ctx, dy0, drki, *_):
+        rk, states = ctx.fwd_states
+        dy = dy0.detach().clone().contiguous() if dy0 is not None else torch.zeros_like(states[0])
+        dk = drki.detach().contiguous() if drki is not None else None
+        for s in range(ctx.nsteps):
+            y = states[s].clone()
+            ctx.solver.tlm_device(ctx.chemstep, rk, y, dy, dk, raise_on_fail=False)
+        return dy
 
     @staticmethod
     def backward(ctx, grad_y):
@@ -54,5 +68,6 @@ class Ros3Integrate(torch.autograd.Function):
 
 
 def integrate(solver, y0, rki, chemstep=5.0, nsteps=1):
-    """y_end after nsteps sync steps; differentiable in y0 and rki."""
+    """y_end after nsteps sync steps; differentiable in y0 and rki (reverse
+    and forward mode)."""
     return Ros3Integrate.apply(y0, rki, solver, chemstep, nsteps)

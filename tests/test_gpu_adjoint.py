@@ -260,3 +260,30 @@ def test_tlm_matches_oracle_and_adjoint(name):
         scale = np.abs(w[:, c]).max() * np.abs(dyg[:, c]).sum() + 1e-300
         assert abs(lhs - rhs) <= 1e-9 * scale, (c, lhs, rhs)
     print(f"tlm {name}: cells {n}, failed {int((nag < 0).sum())}, checked vs oracle {min(48, ok.size)}")
+
+
+def test_forward_mode_autograd_matches_reverse(cb6r5, solver):
+    """torch forward-mode AD through integrate (tangent-linear kernel) against
+    reverse mode (adjoint kernel) over 2 sync steps: w . jvp = grad . tangent."""
+    import torch
+    import torch.autograd.forward_ad as fwAD
+    from cmaq_amd.adjoint import integrate
+    n = 128
+    d = make_cells(cb6r5, np.arange(n), seed=41, workload="mixed")
+    rki = _rates(solver, d)
+    y0 = _dev(d["y"])
+    rng = np.random.default_rng(7)
+    ty = _dev(d["y"] * rng.normal(size=d["y"].shape))
+    tk = rki * _dev(rng.normal(size=tuple(rki.shape)))
+    with fwAD.dual_level():
+        out = integrate(solver, fwAD.make_dual(y0, ty), fwAD.make_dual(rki, tk), 5.0, 2)
+        jv = fwAD.unpack_dual(out).tangent
+    w = _dev(_seed(cb6r5, n))
+    y0g, kg = y0.clone().requires_grad_(True), rki.clone().requires_grad_(True)
+    (integrate(solver, y0g, kg, 5.0, 2) * w).sum().backward()
+    lhs = (w * jv).sum(dim=0).cpu().numpy()
+    rhs = ((y0g.grad * ty).sum(dim=0) + (kg.grad * tk).sum(dim=0)).cpu().numpy()
+    ok = np.isfinite(lhs) & np.isfinite(rhs)
+    assert ok.mean() >= 0.99
+    scale = np.abs(jv.cpu().numpy()).sum(axis=0)[ok] + 1e-300
+    assert (np.abs(lhs[ok] - rhs[ok]) <= 1e-9 * scale).all()
