@@ -125,10 +125,10 @@ def measured_traffic_per_step(mech_name, variant):
     return None, None
 
 
-def row_stream_ceiling():
+def row_stream_ceiling(waves_per_cu):
     """Read throughput of the solver's access stream (scattered 512-byte lane
-    rows from per-wave 2.75 MB regions) measured by cmaq_amd/hbm_rows at 8
-    resident waves per CU, the solver's occupancy; None if the probe is absent."""
+    rows from per-wave 2.75 MB regions) measured by cmaq_amd/hbm_rows at the
+    solver's resident waves per CU (4 or 8); None if the probe is absent."""
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cmaq_amd", "hbm_rows")
     if not os.path.exists(exe):
@@ -140,7 +140,7 @@ def row_stream_ceiling():
     best = None
     for line in r.stdout.splitlines():
         f = line.split()
-        if len(f) == 8 and f[1] == "8" and f[3] == "8":
+        if len(f) == 8 and f[1] == str(waves_per_cu) and f[3] == "8":
             best = float(f[7])
     return best
 
@@ -450,9 +450,11 @@ def main():
         ratio, info = adjoint_leg(args, mech, solver, dev)
         out["fwd_adj_ratio"] = ratio
         out["adjoint"] = info
-    ceiling = row_stream_ceiling()
+    wpc = max(1, solver.query("solve_waves") // max(1, torch.cuda.get_device_properties(dev).multi_processor_count))
+    ceiling = row_stream_ceiling(wpc)
     if ceiling:
         out["roofline"]["row_stream_ceiling"] = ceiling
+        out["roofline"]["row_stream_waves_per_cu"] = wpc
         if traffic:
             out["roofline"]["traffic_frac_of_row_stream_ceiling"] = traffic / solve_s / 1e9 / ceiling
     if not args.no_cpu:

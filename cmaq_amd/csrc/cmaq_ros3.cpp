@@ -502,6 +502,7 @@ extern "C" int cmaq_ros3_query(const cmaq_ros3_ctx *c, const char *key, int64_t 
     else if (!std::strcmp(key, "nrxns")) *value = k->nr;
     else if (!std::strcmp(key, "numb_mech_spc")) *value = k->ns;
     else if (!std::strcmp(key, "grid_waves")) *value = c->waves;
+    else if (!std::strcmp(key, "solve_waves")) *value = (int64_t)c->grid_blocks * (kSolveBlock / kWave);
     else if (!std::strcmp(key, "workspace_doubles_per_lane")) *value = k->nws;
     else return CMAQ_ROS3_E_ARG;
     return 0;

@@ -322,7 +322,8 @@ int cmaq_ros3_set_variant(cmaq_ros3_ctx *ctx, const char *variant);
 const char *cmaq_ros3_variant(const cmaq_ros3_ctx *ctx, int index);
 
 /* Sizes of the selected pattern: key in {"ischan","nnz_day","nnz_night",
- * "lu_ops_day","lu_ops_night","nrxns","numb_mech_spc","grid_waves"}. */
+ * "lu_ops_day","lu_ops_night","nrxns","numb_mech_spc","grid_waves"
+ * (workspace waves),"solve_waves" (waves of the forward kernel's grid)}. */
 int cmaq_ros3_query(const cmaq_ros3_ctx *ctx, const char *key, int64_t *value);
 
 /* Solver species order: inew2old[new] = old (both 0-based), numb_mech_spc entries. */
