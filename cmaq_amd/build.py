@@ -30,9 +30,9 @@ MECHS = {n: n + ".json" for n in (
 # the GPU).  The table keeps the winners; tuning.json records the others that
 # were measured.
 VARIANTS = {
-    "cb6r5_ae7_aq": [(170, 96, "column+k1"), (110, 32, "column+k1")],
+    "cb6r5_ae7_aq": [(170, 96, "tile10x10+k1"), (110, 32, "column+k1")],
     "cracmm2": [(170, 96, "tile12x12+k1")],
-    "cb6r3_ae7_aq": [(150, 64, "column+k1")],
+    "cb6r3_ae7_aq": [(170, 96, "tile10x10+k1")],
     "cb6r5m_ae7_aq": [(150, 64, "column+k1")],
     "racm2_ae6_aq": [(170, 96, "tile12x12+k1")],
     "cracmm1_aq": [(170, 96, "tile12x12+k1")],
