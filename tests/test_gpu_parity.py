@@ -253,3 +253,40 @@ def test_graph_sync_loop_matches_eager(cb6r5, solver_cb6r5):
         yx = _dev(d["y"]).clone()
         solver_cb6r5.run_device(5.0, t["temp"], t["pres"], t["h2o"], t["rj"], yx, raise_on_fail=False)
         torch.cuda.synchronize()
+
+
+def test_full_size_regional_domain(cb6r5, oracle_cb6r5, solver_cb6r5):
+    """The whole 459 x 299 x 35 regional domain (4,803,435 cells) in one
+    call, as the benchmark runs it: a strided sample of 2048 cells against
+    the oracle on the same inputs, and whole-domain invariants (every
+    surviving cell finite and >= CONMIN, failed cells marked -1, counts
+    positive)."""
+    import torch
+    from cmaq_amd.inputs import make_cells_torch
+    ncol = 459 * 299
+    n = ncol * 35
+    d = make_cells_torch(cb6r5, n, seed=0, workload="regional", nlay_cells=ncol, device="cuda")
+    y = d["y"].clone()
+    na = torch.zeros(n, dtype=torch.int32, device="cuda")
+    nj = torch.zeros(n, dtype=torch.int32, device="cuda")
+    solver_cb6r5.run_device(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], y, na, nj, raise_on_fail=False)
+    torch.cuda.synchronize()
+    ok_all = na >= 0
+    assert bool(((na == -1) | (na > 0)).all())
+    assert float(ok_all.float().mean()) > 0.99
+    ys = y[:, ok_all]
+    assert bool(torch.isfinite(ys).all()) and bool((ys >= 1e-30).all())
+    del ys
+    cells = (torch.arange(2048, device="cuda", dtype=torch.int64) * 2345) % n
+    host = {k: d[k][..., cells].cpu().numpy() for k in ("temp", "pres", "h2o", "rj", "y")}
+    ref = oracle_cb6r5.run(5.0, host["temp"], host["pres"], host["h2o"], host["rj"], host["y"], RTOL, ATOL)
+    yg, nag = y[:, cells].cpu().numpy(), na[cells].cpu().numpy()
+    fail_ref = ref["nacc"] < 0
+    assert np.array_equal(fail_ref, nag < 0)
+    ok = ~fail_ref
+    tol = 0.1 * (RTOL * np.abs(ref["y"]) + ATOL)
+    bad = int((np.abs(yg - ref["y"]) > tol)[:, ok].sum())
+    same = float((nag[ok] == ref["nacc"][ok]).mean())
+    print(f"full domain {n} cells: failed {int((~ok_all).sum())}; sample 2048: outside tol {bad}, "
+          f"step match {same:.5f}")
+    assert bad == 0 and same >= 0.999
