@@ -72,6 +72,9 @@ def structure_hash(m):
     return h
 
 
+TLM_MAX = 8   # tangent-linear directions carried by one call (kTlmMaxDirections)
+
+
 def _function(name, params, items, off, budget, dynamic=None, sched=0):
     """HIP source of one generated function from allocator items.
 
@@ -154,7 +157,9 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     for name, size in (("RK", nr), ("RATE", nr), ("Y", ns), ("YP", ns), ("F", ns),
                        ("K1", ns), ("K2", ns), ("K3", ns), ("LU", nnz), ("VD", ns),
                        ("LAM", ns), ("U1", ns), ("U2", ns), ("U3", ns), ("YB", ns),
-                       ("GK", nr), ("S2", nr), ("S3", nr), ("T", 8)):
+                       ("GK", nr), ("S2", nr), ("S3", nr), ("T", 8),
+                       # tangent-linear directions: dY (ns) then dk (nr) per direction
+                       ("TLM", TLM_MAX * (ns + nr))):
         off[name] = pos
         pos += size
     nws = pos
@@ -234,27 +239,35 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
             ("adj_a", adj_a_ops(sym), {"YB", "S2", "S3"}, "", None),
             ("adj_u1", adj_u1_ops(sym), {"U1"}, ", double x21, double x31", None),
             ("adj_b", adj_b_ops(sym), {"LAM", "GK"}, "", None),
-            # tangent-linear direction through one accepted step
-            ("tlm_rhs1", tlm_rhs1_ops(sym), {"U1"}, "", None),
+            # tangent-linear direction through one accepted step; the
+            # direction's dY and dk sit at runtime offsets in the TLM region
+            ("tlm_rhs1", tlm_rhs1_ops(sym), {"U1"}, ", int dy_off_, int dk_off_",
+             {"LAM": "dy_off_", "GK": "dk_off_"}),
             ("msolve", msolve_ops(sym, "X"), {"X"}, ", int x_off_", {"X": "x_off_"}),
-            ("tlm_stage", tlm_stage_ops(sym), {"S2"}, "", None),
-            ("tlm_rhs2", tlm_rhs_ops(sym, 2), {"U2"}, ", double x1", None),
-            ("tlm_rhs3", tlm_rhs_ops(sym, 3), {"U3"}, ", double x1, double x2", None),
-            ("tlm_update", tlm_update_ops(sym), {"LAM"}, "", None)):
+            ("tlm_stage", tlm_stage_ops(sym), {"S2"}, ", int dy_off_, int dk_off_",
+             {"LAM": "dy_off_", "GK": "dk_off_"}),
+            ("tlm_rhs2", tlm_rhs_ops(sym, 2), {"U2"}, ", int dy_off_, int dk_off_, double x1",
+             {"LAM": "dy_off_", "GK": "dk_off_"}),
+            ("tlm_rhs3", tlm_rhs_ops(sym, 3), {"U3"}, ", int dy_off_, int dk_off_, double x1, double x2",
+             {"LAM": "dy_off_", "GK": "dk_off_"}),
+            ("tlm_update", tlm_update_ops(sym), {"LAM"}, ", int dy_off_", {"LAM": "dy_off_"})):
         items, cnt = allocate(ops, outs, budget, lookahead, entry_all)
         stats[fname] = _stat(ops, cnt)
         body += fn(fname, params, items, off, budget, dyn)
         b("")
-    b("__device__ __noinline__ void tlm_step(double *base, double dtinv)")
+    b("__device__ __noinline__ void tlm_step(double *base, double dtinv, int ndir)")
     b("{")
-    b("    tlm_rhs1(base);")
-    b("    msolve(base, OFF_U1);")
-    b("    tlm_stage(base);")
-    b("    tlm_rhs2(base, C21 * dtinv);")
-    b("    msolve(base, OFF_U2);")
-    b("    tlm_rhs3(base, C31 * dtinv, C32 * dtinv);")
-    b("    msolve(base, OFF_U3);")
-    b("    tlm_update(base);")
+    b("    for (int p = 0; p < ndir; ++p) {")
+    b("        const int dy = OFF_TLM + p * TLM_RS, dk = dy + NS;")
+    b("        tlm_rhs1(base, dy, dk);")
+    b("        msolve(base, OFF_U1);")
+    b("        tlm_stage(base, dy, dk);")
+    b("        tlm_rhs2(base, dy, dk, C21 * dtinv);")
+    b("        msolve(base, OFF_U2);")
+    b("        tlm_rhs3(base, dy, dk, C31 * dtinv, C32 * dtinv);")
+    b("        msolve(base, OFF_U3);")
+    b("        tlm_update(base, dy);")
+    b("    }")
     b("}")
     b("")
 
@@ -266,6 +279,8 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w("#include \"../ros3_device.h\"")
     w(f"namespace ros3 {{ namespace {cname} {{")
     w(f"constexpr int NS = {ns}, NR = {nr}, NNZ = {nnz}, NWS = {nws};")
+    w(f"constexpr int TLM_MAX = {TLM_MAX}, TLM_RS = NS + NR;   // tangent-linear directions per call")
+    w("static_assert(TLM_MAX == kTlmMaxDirections, \"TLM_MAX\");")
     w(f"constexpr int REG_BUDGET = {budget}, WAVES_PER_EU = {waves_per_eu(budget)};")
     w(f"constexpr unsigned long long STRUCT_HASH = 0x{h:016x}ull;")
     for k, v in off.items():
@@ -327,16 +342,24 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w("{")
     w("    ROS3_PROLOGUE")
     w("    const int64_t n = P.stride;")
-    w("    for (int r = 0; r < NR; ++r) ST(OFF_GK + r, P.dk ? P.dk[r * n + cell] : 0.0);")
+    w("    for (int p = 0; p < P.ndir; ++p) {")
+    w("        const int o = OFF_TLM + p * TLM_RS;")
+    w("        const double *dy = P.lam + (int64_t)p * NS * n + cell;")
+    w("        for (int r = 0; r < NR; ++r) ST(o + NS + r, P.dk ? P.dk[((int64_t)p * NR + r) * n + cell] : 0.0);")
     for i in range(ns):
-        w(f"    ST(OFF_LAM + {i}, P.lam[{sym.inew2old[i]} * n + cell]);")
+        w(f"        ST(o + {i}, dy[{sym.inew2old[i]} * n]);")
+    w("    }")
     w("}")
     w("__device__ __noinline__ void store_tlm_cell(const SolveParams &P, int64_t cell, double *base_, bool failed)")
     w("{")
     w("    ROS3_PROLOGUE")
     w("    const int64_t n = P.stride;")
+    w("    for (int p = 0; p < P.ndir; ++p) {")
+    w("        const int o = OFF_TLM + p * TLM_RS;")
+    w("        double *dy = P.lam + (int64_t)p * NS * n + cell;")
     for i in range(ns):
-        w(f"    P.lam[{sym.inew2old[i]} * n + cell] = failed ? __builtin_nan(\"\") : LD(OFF_LAM + {i});")
+        w(f"        dy[{sym.inew2old[i]} * n] = failed ? __builtin_nan(\"\") : LD(o + {i});")
+    w("    }")
     w("}")
     w("__device__ __noinline__ void tape_store(double *base_, double *rec, double h)")
     w("{")

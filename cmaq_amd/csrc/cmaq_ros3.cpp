@@ -621,7 +621,7 @@ static int check_fail(cmaq_ros3_ctx *c, const unsigned char *small, cmaq_ros3_fa
 // ev_from is the event recorded before the rates (or before the solve).
 static int launch_solve(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki, double *y,
                         int32_t *nacc, int32_t *nrej, cmaq_ros3_fail *fail, hipStream_t s,
-                        double *dy = nullptr, const double *dk = nullptr)
+                        double *dy = nullptr, const double *dk = nullptr, int32_t ndir = 0)
 {
     HIP_TRY(c, hipEventRecord(c->ev[1], s));
     SolveParams P;
@@ -631,6 +631,7 @@ static int launch_solve(cmaq_ros3_ctx *c, int64_t n, double chemstep, const doub
     if (dy) {   // the forward with one tangent-linear direction
         P.lam = dy;
         P.dk = dk;
+        P.ndir = ndir;
         HIP_TRY(c, hipLaunchKernel(c->mk->tlm_kernel, dim3(c->grid_tlm), dim3(kSolveBlock), args, 0, s));
     } else {
         HIP_TRY(c, hipLaunchKernel(c->mk->solve_kernel, dim3(c->grid_blocks), dim3(kSolveBlock), args, 0, s));
@@ -696,17 +697,19 @@ extern "C" int cmaq_ros3_solve_device(cmaq_ros3_ctx *c, int64_t n, double chemst
 }
 
 extern "C" int cmaq_ros3_tlm_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,
-                                    double *y, int32_t *nacc, int32_t *nrej, double *dy, const double *dk,
-                                    cmaq_ros3_fail *fail, void *stream)
+                                    double *y, int32_t *nacc, int32_t *nrej, int32_t ndir, double *dy,
+                                    const double *dk, cmaq_ros3_fail *fail, void *stream)
 {
     if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (ndir < 1 || ndir > kTlmMaxDirections)
+        return set_err(c, CMAQ_ROS3_E_ARG, "ndir must be 1.." + std::to_string(kTlmMaxDirections));
     if (n > 0 && (!rki || !y || !dy)) return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
     if (n == 0) return 0;
     HIP_TRY(c, hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
     HIP_TRY(c, hipEventRecord(c->ev[0], s));
-    return launch_solve(c, n, chemstep, rki, y, nacc, nrej, fail, s, dy, dk);
+    return launch_solve(c, n, chemstep, rki, y, nacc, nrej, fail, s, dy, dk, ndir);
 }
 
 extern "C" int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,

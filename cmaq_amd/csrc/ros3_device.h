@@ -233,6 +233,8 @@ __global__ __launch_bounds__(256) void rates_kernel(RatesParams P)
 }
 
 // ---- persistent integration kernel ------------------------------------
+constexpr int kTlmMaxDirections = 8;   // tangent-linear directions per call
+
 struct SolveParams {
     int64_t ncells;           // cells of this launch: cell0 .. cell0 + ncells - 1,
     int64_t cell0;            // or order[0 .. ncells - 1] when order is set
@@ -257,9 +259,10 @@ struct SolveParams {
     const int64_t *tape_off;  // [stride] first record of each cell
     const int32_t *nacc_fwd;  // [stride] accepted steps of the forward run
     double *lam;              // [ns][stride] mechanism order: dL/dy_end in, dL/dy0 out;
-                              // tangent-linear run: dy0 in, dy_end out
+                              // tangent-linear run: [ndir][ns][stride] dy0 in, dy_end out
     double *grad_rki;         // [nr][stride] accumulated dL/drki, or nullptr
-    const double *dk;         // tangent-linear run: [nr][stride] rate direction, or nullptr
+    const double *dk;         // tangent-linear run: [ndir][nr][stride] rate directions, or nullptr
+    int32_t ndir;             // tangent-linear run: directions (1..kTlmMaxDirections)
 };
 
 __device__ inline int lane_id() { return __lane_id(); }

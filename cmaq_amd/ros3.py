@@ -104,7 +104,8 @@ def lib():
         L.cmaq_ros3_set_variant.argtypes = [vp, C.c_char_p]
         L.cmaq_ros3_variant.argtypes = [vp, C.c_int]
         L.cmaq_ros3_variant.restype = C.c_char_p
-        L.cmaq_ros3_tlm_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 6 + [C.POINTER(_Fail), vp]
+        L.cmaq_ros3_tlm_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 4 + [C.c_int32] + [vp] * 2 + \
+            [C.POINTER(_Fail), vp]
         L.cmaq_ros3_set_cgrid_map.argtypes = [vp, C.c_int32, _i32p, _i32p, _f64p]
         L.cmaq_ros3_chem.argtypes = [vp, C.POINTER(_Grid), C.c_double, _i32p, _i32p, C.POINTER(_Fail)]
         L.cmaq_ros3_chem_device.argtypes = [vp, C.POINTER(_Grid), C.c_double, vp, vp, C.POINTER(_Fail), vp]
@@ -453,18 +454,22 @@ class Ros3Solver:
             raise self._err(rc, fail)
         return fail if rc != 0 else None
 
+    TLM_MAX_DIRECTIONS = 8
+
     def tlm_device(self, chemstep, rki, y, dy, dk=None, nacc=None, nrej=None, stream=None, raise_on_fail=True):
-        """solve_device plus one tangent-linear direction: y [ns, n] advances
-        in place, dy [ns, n] (dy0 in, d y_end out) along dk [nr, n] (optional)."""
+        """solve_device plus tangent-linear directions: y [ns, n] advances in
+        place, dy [ndir, ns, n] or [ns, n] (dy0 in, d y_end out) along dk
+        [ndir, nr, n] or [nr, n] (optional); ndir <= 8."""
         import torch
         n = y.numel() // self.ns
-        self._check([("rki", rki, self.nr), ("y", y, self.ns), ("dy", dy, self.ns), ("dk", dk, self.nr),
-                     ("nacc", nacc, 1), ("nrej", nrej, 1)], n)
+        ndir = dy.numel() // (self.ns * n) if n else 1
+        self._check([("rki", rki, self.nr), ("y", y, self.ns), ("dy", dy, ndir * self.ns),
+                     ("dk", dk, ndir * self.nr), ("nacc", nacc, 1), ("nrej", nrej, 1)], n)
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         f = _Fail()
         rc = lib().cmaq_ros3_tlm_device(self._ctx, n, chemstep, _ptr(rki), _ptr(y), _ptr(nacc), _ptr(nrej),
-                                        _ptr(dy), _ptr(dk), C.byref(f), C.c_void_p(stream))
+                                        ndir, _ptr(dy), _ptr(dk), C.byref(f), C.c_void_p(stream))
         fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
         if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
             raise self._err(rc, fail)

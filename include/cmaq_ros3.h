@@ -189,21 +189,23 @@ int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep
                              double *grad_rki, cmaq_ros3_fail *fail, void *stream);
 
 /*
- * Tangent-linear (forward-mode) sensitivity along one direction, carried
- * through a cmaq_ros3_solve_device call (no counterpart in the reference; an
- * exact alternative to DDM-3D's Crank-Nicolson sensitivity step,
- * ddm3d/DDM3D_CHEM.F:155-511).  Integrates exactly as cmaq_ros3_solve_device
- * (same steps, same y, n_accepted, n_rejected) and, with each accepted
- * step's own stages and factorization, advances dy [ns][ncells] (mechanism
- * order, in/out: dy0 in, d y_end out) along the rate-constant direction dk
- * [nrxns][ncells] (NULL = 0).  The accepted step sizes are held fixed, as in
- * cmaq_ros3_adjoint_device, so  w . dy_end = lambda0 . dy0 + grad_rki . dk.
- * Cells that fail get NaN in dy.  Several directions: one call each, from the
- * same y0.  Device memory, `stream` as in cmaq_ros3_run_device.
+ * Tangent-linear (forward-mode) sensitivities along ndir directions (1..8),
+ * carried through a cmaq_ros3_solve_device call (no counterpart in the
+ * reference; an exact alternative to DDM-3D's Crank-Nicolson sensitivity
+ * step, ddm3d/DDM3D_CHEM.F:155-511, whose parameters are the directions).
+ * Integrates exactly as cmaq_ros3_solve_device (same steps, same y,
+ * n_accepted, n_rejected) and, with each accepted step's own stages and
+ * factorization, advances dy [ndir][ns][ncells] (mechanism order, in/out:
+ * dy0 in, d y_end out) along the rate-constant directions dk
+ * [ndir][nrxns][ncells] (NULL = 0).  The accepted step sizes are held fixed,
+ * as in cmaq_ros3_adjoint_device, so for each direction
+ * w . dy_end = lambda0 . dy0 + grad_rki . dk.  Cells that fail get NaN in dy.
+ * More than 8 directions: further calls from the same y0.  Device memory,
+ * `stream` as in cmaq_ros3_run_device.
  */
 int cmaq_ros3_tlm_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min,
                          const double *rki, double *y_ppm, int32_t *n_accepted,
-                         int32_t *n_rejected, double *dy, const double *dk,
+                         int32_t *n_rejected, int32_t ndir, double *dy, const double *dk,
                          cmaq_ros3_fail *fail, void *stream);
 
 /*

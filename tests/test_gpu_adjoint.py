@@ -218,9 +218,10 @@ def test_adjoint_matches_oracle_cracmm2(cracmm2, oracle_cracmm2):
 
 @pytest.mark.parametrize("name", ["cb6r5_ae7_aq", "saprc07tic_ae7i_aq"])
 def test_tlm_matches_oracle_and_adjoint(name):
-    """cmaq_ros3_tlm_device: the forward it carries equals solve_device bit
-    for bit; the tangent along (dy0, dk) matches the oracle's om_tlm_cell
-    (1e-8 of the cell's largest entry) and, on the GPU alone, the duality
+    """cmaq_ros3_tlm_device with three directions at once: the forward it
+    carries equals solve_device bit for bit; each direction equals a
+    one-direction call bit for bit, matches the oracle's om_tlm_cell (1e-8 of
+    the cell's largest entry) and, on the GPU alone, the duality
     w . dy_end = lambda0 . dy0 + grad_rki . dk with the adjoint."""
     import torch
     from cmaq_amd.ros3 import Ros3Solver, load_mechanism
@@ -231,10 +232,14 @@ def test_tlm_matches_oracle_and_adjoint(name):
     n = 256
     d = make_cells(mech, np.arange(n), seed=31, workload="mixed")
     rki = _rates(s, d)
-    rng = np.random.default_rng(3)
-    dy0 = d["y"] * rng.normal(size=d["y"].shape)
     kk = rki.cpu().numpy()
-    dk = kk * rng.normal(size=kk.shape)
+    rng = np.random.default_rng(3)
+    names = mech["chemistry_spc"]
+    ino2 = names.index("NO2")
+    dy0 = np.stack([d["y"] * rng.normal(size=d["y"].shape),
+                    np.where(np.arange(len(names))[:, None] == ino2, d["y"], 0.0),
+                    np.zeros_like(d["y"])])
+    dk = np.stack([kk * rng.normal(size=kk.shape), np.zeros_like(kk), kk * (rng.uniform(size=kk.shape) < 0.3)])
     y_f, na_f = _dev(d["y"]).clone(), torch.zeros(n, dtype=torch.int32, device="cuda")
     s.solve_device(5.0, rki, y_f, na_f, raise_on_fail=False)
     y_t, na_t = _dev(d["y"]).clone(), torch.zeros(n, dtype=torch.int32, device="cuda")
@@ -242,24 +247,32 @@ def test_tlm_matches_oracle_and_adjoint(name):
     s.tlm_device(5.0, rki, y_t, dy, _dev(dk), nacc=na_t, raise_on_fail=False)
     assert torch.equal(y_f, y_t) and torch.equal(na_f, na_t)
     dyg, nag = dy.cpu().numpy(), na_t.cpu().numpy()
-    assert np.isnan(dyg[:, nag < 0]).all()
+    assert np.isnan(dyg[:, :, nag < 0]).all()
+    for p in range(3):   # one direction per call gives the same bits
+        y1, d1 = _dev(d["y"]).clone(), _dev(dy0[p]).clone()
+        s.tlm_device(5.0, rki, y1, d1, _dev(dk[p]), raise_on_fail=False)
+        assert np.array_equal(d1.cpu().numpy(), dyg[p], equal_nan=True), p
     w = _seed(mech, n)
     lam = _dev(w).clone()
     gk = torch.zeros_like(rki)
     s.adjoint_device(5.0, rki, _dev(d["y"]), na_t, lam, gk)
     lg, gg = lam.cpu().numpy(), gk.cpu().numpy()
     ok = np.flatnonzero(nag >= 0)
-    for c in ok[:48]:
+    for c in ok[:32]:
         h, _ = o.steps_cell(5.0, kk[:, c], d["y"][:, c])
         assert len(h) == nag[c]
-        ref = o.tlm_cell(kk[:, c], d["y"][:, c], h, dy0[:, c], dk[:, c])
-        assert np.abs(dyg[:, c] - ref).max() <= 1e-8 * np.abs(ref).max(), c
+        for p in range(3):
+            ref = o.tlm_cell(kk[:, c], d["y"][:, c], h, dy0[p, :, c], dk[p, :, c])
+            assert np.abs(dyg[p, :, c] - ref).max() <= 1e-8 * np.abs(ref).max(), (c, p)
     for c in ok:
-        lhs = w[:, c] @ dyg[:, c]
-        rhs = lg[:, c] @ dy0[:, c] + gg[:, c] @ dk[:, c]
-        scale = np.abs(w[:, c]).max() * np.abs(dyg[:, c]).sum() + 1e-300
-        assert abs(lhs - rhs) <= 1e-9 * scale, (c, lhs, rhs)
-    print(f"tlm {name}: cells {n}, failed {int((nag < 0).sum())}, checked vs oracle {min(48, ok.size)}")
+        for p in range(3):
+            lhs = w[:, c] @ dyg[p, :, c]
+            rhs = lg[:, c] @ dy0[p, :, c] + gg[:, c] @ dk[p, :, c]
+            scale = np.abs(w[:, c]).max() * np.abs(dyg[p, :, c]).sum() + 1e-300
+            assert abs(lhs - rhs) <= 1e-9 * scale, (c, p, lhs, rhs)
+    with pytest.raises(Exception):
+        s.tlm_device(5.0, rki, _dev(d["y"]).clone(), _dev(np.zeros((9,) + d["y"].shape)))
+    print(f"tlm {name}: cells {n}, 3 directions, failed {int((nag < 0).sum())}, checked vs oracle {min(32, ok.size)}")
 
 
 def test_forward_mode_autograd_matches_reverse(cb6r5, solver):
