@@ -99,15 +99,16 @@ def generate(names=None):
         for (b, l, order) in VARIANTS[name]:
             var = variant_name(b, l, order)
             cname = f"{name}_{var}"
-            text, _ = emit(mech, cname, budget=b, lookahead=l, variant=var, is_default=(var == dflt),
-                           refill_min=refill, jac_order=order)
-            path = os.path.join(GEN, f"ros3_{cname}.hip")
-            _write_if_changed(path, text)
-            srcs.append((path, KERNEL_FLAGS))
+            (text, text_adj), _ = emit(mech, cname, budget=b, lookahead=l, variant=var,
+                                       is_default=(var == dflt), refill_min=refill, jac_order=order)
+            for suffix, t in (("", text), ("_adj", text_adj)):
+                path = os.path.join(GEN, f"ros3_{cname}{suffix}.hip")
+                _write_if_changed(path, t)
+                srcs.append((path, KERNEL_FLAGS))
             cnames.append(cname)
     if set(names) == set(MECHS):
         # drop sources and objects of variants no longer in the table
-        keep = {f"ros3_{c}" for c in cnames}
+        keep = {f"ros3_{c}" for c in cnames} | {f"ros3_{c}_adj" for c in cnames}
         for d, ext in ((GEN, ".hip"), (OBJ, ".o")):
             for f in os.listdir(d) if os.path.isdir(d) else []:
                 stem = f[:f.index(".")] if "." in f else f
@@ -159,6 +160,8 @@ def build(names=None, verbose=True, force=False):
         base = os.path.splitext(os.path.basename(src))[0]
         jobs.append((src, os.path.join(OBJ, base + ".o"), src.endswith("registry.cpp"), extra, force))
     jobs.append((os.path.join(CSRC, "cmaq_ros3.cpp"), os.path.join(OBJ, "cmaq_ros3.o"), False, (), force))
+    # largest sources first, so that the longest compiles overlap the others
+    jobs.sort(key=lambda j: -os.path.getsize(j[0]))
     workers = int(os.environ.get("MAX_JOBS", "8"))
     with cf.ThreadPoolExecutor(max_workers=max(1, min(workers, len(jobs)))) as ex:
         objs = list(ex.map(lambda j: _compile(*j), jobs))

@@ -1,16 +1,22 @@
 """Emit the fully unrolled gfx950 ROS3 kernels for one mechanism.
 
-The generated translation unit holds, for the mechanism's day pattern,
-straight-line device functions built from programs.py's operation lists:
+Two translation units are generated per mechanism and variant, so that the
+halves of a large mechanism compile in parallel.  The first holds, for the
+mechanism's day pattern, straight-line device functions built from
+programs.py's operation lists:
 
 * ``feval``    -- f(y) in RBFEVAL's order (rbfeval.F:93-149);
 * ``jac_lu``   -- Jacobian, diagonal shift and sparse LU (rbjacob.F,
   rbsolver.F:377-381, rbdecomp.F);
 * ``solve1..3`` -- the three stage solves (rbsolve.F) with the stage
   right-hand sides of rbsolver.F:386-422 formed in place;
-* the persistent solver kernel (ros3_solver_kernel.inc) that calls them;
-* the discrete adjoint of one accepted step (programs.py: yp_ops ..
-  adj_b_ops) and the persistent reverse kernel (ros3_adjoint_kernel.inc).
+* the tangent-linear step (programs.py: tlm_rhs1_ops .. tlm_update_ops);
+* the persistent solver kernel (ros3_solver_kernel.inc) that calls them, in
+  its forward, tape-replay and tangent-linear forms, and the registry record.
+The second holds the discrete adjoint of one accepted step (programs.py:
+yp_ops .. adj_b_ops) and the persistent reverse kernel
+(ros3_adjoint_kernel.inc), whose host stub the registry record of the first
+refers to.
 
 Registers are assigned by regalloc.allocate: at most REG_BUDGET doubles of a
 lane's workspace are register-resident, loads are issued LOOKAHEAD operations
@@ -213,7 +219,8 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
         ops = jac_lu_ops(sym, base_order, fuse_k1)
         items, cnt = allocate(ops, {"LU", "VD", "K1"} if fuse_k1 else {"LU", "VD"}, budget, lookahead, entry_all, lds)
         stats["jac_lu"] = _stat(ops, cnt)
-        body += fn("jac_lu", ", double gdtinv", items, off, budget)
+        jac_lu_text = fn("jac_lu", ", double gdtinv", items, off, budget)
+        body += jac_lu_text
         b("")
         for stage in (1, 2, 3):
             ops = stage1_backward() if stage == 1 else solve_ops(sym, stage)
@@ -223,14 +230,21 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
             params = "" if stage == 1 else (", double x1" if stage == 2 else ", double x1, double x2")
             body += fn(f"solve{stage}", params, items, off, budget)
             b("")
+    # the reverse kernel and its functions form a second translation unit so
+    # that the two halves of a large mechanism compile in parallel
+    body_adj = []
     if fuse_k1:
         # the adjoint's refactorization has no f(Y) to substitute
         ops = jac_lu_ops(sym, base_order)
         items, cnt = allocate(ops, {"LU", "VD"}, budget, lookahead, entry_all)
-        body += fn("jac_lu_plain", ", double gdtinv", items, off, budget)
-        b("")
+        body_adj += fn("jac_lu_plain", ", double gdtinv", items, off, budget)
+        body_adj.append("")
+    elif not merge:
+        body_adj += jac_lu_text
+        body_adj.append("")
 
-    # discrete adjoint of one accepted step
+    # discrete adjoint of one accepted step (second translation unit), then
+    # the tangent-linear step (first)
     for fname, ops, outs, params, dyn in (
             ("yp_form", yp_ops(sym), {"YP"}, "", None),
             ("adj_seed", adj_seed_ops(sym), {"LAM", "U3"}, "", None),
@@ -253,8 +267,9 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
             ("tlm_update", tlm_update_ops(sym), {"LAM"}, ", int dy_off_", {"LAM": "dy_off_"})):
         items, cnt = allocate(ops, outs, budget, lookahead, entry_all)
         stats[fname] = _stat(ops, cnt)
-        body += fn(fname, params, items, off, budget, dyn)
-        b("")
+        dest = body if fname.startswith(("tlm_", "msolve")) else body_adj
+        dest += fn(fname, params, items, off, budget, dyn)
+        dest.append("")
     b("__device__ __noinline__ void tlm_step(double *base, double dtinv, int ndir)")
     b("{")
     b("    for (int p = 0; p < ndir; ++p) {")
@@ -274,6 +289,7 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     out = []
     w = out.append
     w(f"// Generated by cmaq_amd/codegen/emit_hip.py for {m['mechname']}; do not edit.")
+    w("// @UNIT@")
     w(f"// register budget {budget} doubles, load lookahead {lookahead} operations, LU schedule {jac_order};")
     w("// per call (operations, loads, stores): " + ", ".join(f"{k} {v}" for k, v in stats.items()))
     w("#include \"../ros3_device.h\"")
@@ -300,7 +316,47 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     if lds:
         w(f"constexpr int LDS_SLOTS = {lds};")
         w("__shared__ double lds_tier[(256 / kWave) * LDS_SLOTS * kWave];   // 4 waves per workgroup")
+    w(f"constexpr int TAPE_RS = {4 * ns + 2};   // doubles per adjoint tape record")
     w("")
+    header = list(out)
+    # ---- second unit: the reverse sweep of the discrete adjoint ----
+    adj = [line.replace("// @UNIT@", "// unit 2 of 2: discrete adjoint (reverse_kernel)") for line in header
+           if not line.startswith(("constexpr int LDS_SLOTS", "__shared__ double lds_tier"))]
+    wa = adj.append
+    adj += body_adj
+    # adjoint cell load / store; tape records (one per accepted step: Y, K1,
+    # K2, K3 and the step size, contiguous per cell) are written by unit 1
+    wa("__device__ __noinline__ void load_adj_cell(const SolveParams &P, int64_t cell, double *base_)")
+    wa("{")
+    wa("    ROS3_PROLOGUE")
+    wa("    const int64_t n = P.stride;")
+    wa("    for (int r = 0; r < NR; ++r) { ST(OFF_RK + r, P.rki[r * n + cell]); ST(OFF_GK + r, 0.0); }")
+    for i in range(ns):
+        wa(f"    ST(OFF_LAM + {i}, P.lam[{sym.inew2old[i]} * n + cell]);")
+    wa("}")
+    wa("__device__ __noinline__ void store_adj_cell(const SolveParams &P, int64_t cell, double *base_)")
+    wa("{")
+    wa("    ROS3_PROLOGUE")
+    wa("    const int64_t n = P.stride;")
+    for i in range(ns):
+        wa(f"    P.lam[{sym.inew2old[i]} * n + cell] = LD(OFF_LAM + {i});")
+    wa("    if (P.grad_rki)")
+    wa("        for (int r = 0; r < NR; ++r) P.grad_rki[r * n + cell] += LD(OFF_GK + r);")
+    wa("}")
+    wa("__device__ __noinline__ double tape_load(double *base_, const double *rec)")
+    wa("{")
+    wa("    ROS3_PROLOGUE")
+    for k, arr in enumerate(("Y", "K1", "K2", "K3")):
+        wa(f"    for (int i = 0; i < NS; ++i) ST(OFF_{arr} + i, rec[{k} * NS + i]);")
+    wa("    return rec[4 * NS];")
+    wa("}")
+    wa(f"#define ROS3_JAC_LU_PLAIN {'jac_lu_plain' if fuse_k1 else 'jac_lu'}")
+    wa("#include \"../ros3_adjoint_kernel.inc\"")
+    for u in ("LD", "ST", "ROS3_PROLOGUE", "ROS3_JAC_LU_PLAIN"):
+        wa(f"#undef {u}")
+    wa(f"}} }} // namespace ros3::{cname}")
+    # ---- first unit: forward, tape replay and tangent-linear kernels ----
+    out[1] = "// unit 1 of 2: forward, replay and tangent-linear kernels; registry record"
     out += body
     # cell load / store: inputs and results are cell-fastest global arrays
     w("__device__ __noinline__ void load_cell(const SolveParams &P, int64_t cell, double *base_)")
@@ -317,26 +373,6 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w("    const int64_t n = P.stride;")
     for i in range(ns):
         w(f"    P.y[{sym.inew2old[i]} * n + cell] = LD(OFF_Y + {i});")
-    w("}")
-    # adjoint cell load / store and step tapes (one record per accepted step:
-    # Y, K1, K2, K3 and the step size, contiguous per cell)
-    w(f"constexpr int TAPE_RS = {4 * ns + 2};")
-    w("__device__ __noinline__ void load_adj_cell(const SolveParams &P, int64_t cell, double *base_)")
-    w("{")
-    w("    ROS3_PROLOGUE")
-    w("    const int64_t n = P.stride;")
-    w("    for (int r = 0; r < NR; ++r) { ST(OFF_RK + r, P.rki[r * n + cell]); ST(OFF_GK + r, 0.0); }")
-    for i in range(ns):
-        w(f"    ST(OFF_LAM + {i}, P.lam[{sym.inew2old[i]} * n + cell]);")
-    w("}")
-    w("__device__ __noinline__ void store_adj_cell(const SolveParams &P, int64_t cell, double *base_)")
-    w("{")
-    w("    ROS3_PROLOGUE")
-    w("    const int64_t n = P.stride;")
-    for i in range(ns):
-        w(f"    P.lam[{sym.inew2old[i]} * n + cell] = LD(OFF_LAM + {i});")
-    w("    if (P.grad_rki)")
-    w("        for (int r = 0; r < NR; ++r) P.grad_rki[r * n + cell] += LD(OFF_GK + r);")
     w("}")
     w("__device__ __noinline__ void load_tlm_cell(const SolveParams &P, int64_t cell, double *base_)")
     w("{")
@@ -368,32 +404,32 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
         w(f"    for (int i = 0; i < NS; ++i) rec[{k} * NS + i] = LD(OFF_{arr} + i);")
     w("    rec[4 * NS] = h;")
     w("}")
-    w("__device__ __noinline__ double tape_load(double *base_, const double *rec)")
-    w("{")
-    w("    ROS3_PROLOGUE")
-    for k, arr in enumerate(("Y", "K1", "K2", "K3")):
-        w(f"    for (int i = 0; i < NS; ++i) ST(OFF_{arr} + i, rec[{k} * NS + i]);")
-    w("    return rec[4 * NS];")
-    w("}")
     w(f"const int32_t INEW2OLD_HOST[NS] = {{{', '.join(str(v) for v in sym.inew2old)}}};")
     w("")
-    w(f"#define ROS3_JAC_LU_PLAIN {'jac_lu_plain' if fuse_k1 else 'jac_lu'}")
     if merge:
         w("#define ROS3_MERGED_STEP 1")
     w("#include \"../ros3_solver_kernel.inc\"")
-    w("#include \"../ros3_adjoint_kernel.inc\"")
+    w("__global__ __launch_bounds__(256, WAVES_PER_EU) void reverse_kernel(SolveParams P);   // unit 2")
     w("#undef LD")
     w("#undef ST")
     w("#undef ROS3_PROLOGUE")
-    w("#undef ROS3_JAC_LU_PLAIN")
     w("#undef ROS3_MERGED_STEP")
     w(f"}} }} // namespace ros3::{cname}")
     w("")
+    # the record's kernel addresses make the device pass instantiate the
+    # forward kernels; reverse_kernel's device code lives in unit 2, so only
+    # the host pass takes its address
+    w("#ifdef __HIP_DEVICE_COMPILE__")
+    w("#define ROS3_REVERSE_KERNEL nullptr")
+    w("#else")
+    w(f"#define ROS3_REVERSE_KERNEL (const void *)&ros3::{cname}::reverse_kernel")
+    w("#endif")
     w(f"extern \"C\" const ros3::MechKernels ros3_kernels_{cname} = {{")
     w(f"    \"{m['mechname']}\", ros3::{cname}::STRUCT_HASH, ros3::{cname}::NS, ros3::{cname}::NR,")
     w(f"    ros3::{cname}::NNZ, {len(day.lu_ops)}, {sym.night.nnz}, {len(sym.night.lu_ops)}, ros3::{cname}::NWS,")
     w(f"    (const void *)&ros3::{cname}::solve_kernel_t<false>, ros3::{cname}::INEW2OLD_HOST,")
     w(f"    \"{variant}\", {budget}, {lookahead}, ros3::{cname}::WAVES_PER_EU, {1 if is_default else 0}, {refill_min},")
-    w(f"    (const void *)&ros3::{cname}::solve_kernel_t<true>, (const void *)&ros3::{cname}::reverse_kernel,")
+    w(f"    (const void *)&ros3::{cname}::solve_kernel_t<true>, ROS3_REVERSE_KERNEL,")
     w(f"    ros3::{cname}::TAPE_RS, (const void *)&ros3::{cname}::solve_kernel_t<false, true>}};")
-    return "\n".join(out) + "\n", sym
+    w("#undef ROS3_REVERSE_KERNEL")
+    return ("\n".join(out) + "\n", "\n".join(adj) + "\n"), sym
