@@ -327,16 +327,20 @@ int dalloc(cmaq_ros3_ctx *c, T **p, size_t n)
 }  // namespace
 
 // Grid and workspace for one kernel variant: as many resident workgroups as
-// the variant's occupancy allows (or the caller's waves-per-CU cap), one
-// lane-private workspace per resident lane.
+// the variant's occupancy allows, one lane-private workspace per resident
+// lane.  The caller's waves-per-CU cap limits the whole grid to cap x CUs
+// waves in whole workgroups, so a cap below one workgroup per CU leaves CUs
+// idle (fewer concurrent workspace streams).
 static int grid_for(cmaq_ros3_ctx *c, const void *kernel, int *grid)
 {
     int per_cu = 0;
     HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kSolveBlock, 0));
     if (per_cu < 1) per_cu = 1;
     const int wpb = kSolveBlock / kWave;
-    if (c->waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, c->waves_per_cu / wpb));
-    *grid = c->num_cus * per_cu;
+    int64_t g = (int64_t)c->num_cus * per_cu;
+    if (c->waves_per_cu > 0)
+        g = std::min<int64_t>(g, std::max<int64_t>(1, (int64_t)c->waves_per_cu * c->num_cus / wpb));
+    *grid = (int)g;
     return 0;
 }
 

@@ -63,7 +63,7 @@ int main(int argc, char **argv)
     hipDeviceProp_t prop;
     CHECK(hipGetDeviceProperties(&prop, 0));
     const int cus = prop.multiProcessorCount;
-    for (int wpc : {4, 8, 16}) {
+    for (int wpc : {1, 2, 4, 8, 16}) {
         const long waves = (long)cus * wpc;
         const size_t bytes = (size_t)waves * kRowsPerWave * kWave * 2 * sizeof(double);
         double *buf = nullptr, *out = nullptr;

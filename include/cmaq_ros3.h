@@ -76,7 +76,9 @@ typedef struct cmaq_ros3_mech {
 typedef struct cmaq_ros3_opts {
     int32_t device;             /* HIP device ordinal; -1 = current device        */
     int32_t max_attempts;       /* per cell and call; 0 = 1000000                 */
-    int32_t waves_per_cu;       /* resident waves per CU; 0 = autotuned / max     */
+    int32_t waves_per_cu;       /* cap on the solver grid: at most waves_per_cu x
+                                   CUs waves, in workgroups of 4; 0 = as many as
+                                   the kernel's occupancy allows                  */
     int32_t refill_min;         /* lanes that must be idle before a wave refills;
                                    0 = autotuned                                  */
 } cmaq_ros3_opts;
