@@ -24,7 +24,9 @@ The JSON line also carries
                  differentiation workload: 262,144 mixed-regime cells, 12
                  sync steps of 5 min, loss = sum(O3 + 0.1 NO2) at the end,
                  gradients with respect to all initial concentrations and
-                 all rate constants (details under "adjoint").
+                 all rate constants, every accepted step stored in the
+                 forward (details under "adjoint"; the mode that recomputes
+                 the stages per sync step under "adjoint.replay_mode").
 """
 
 import argparse
@@ -58,6 +60,8 @@ def _args():
     p.add_argument("--adj-svoc", action="store_true",
                    help="adjoint leg: VOC/SVOC species log-uniform on [1e-5, 5] ppb (the CRACMM2 workload)")
     p.add_argument("--no-adj", action="store_true")
+    p.add_argument("--adj-tape", default="stored", choices=["stored", "replay"],
+                   help="adjoint leg's headline mode: every accepted step stored, or stages recomputed")
     p.add_argument("--loop-cells", type=int, default=1048576)
     p.add_argument("--loop-sync-steps", type=int, default=12)
     p.add_argument("--no-loop", action="store_true")
@@ -203,10 +207,10 @@ def adjoint_leg(args, mech, solver, dev):
             att += int(na.clamp(min=0).sum())
         return att
 
-    def fwd_adj():
+    def fwd_adj(tape):
         y0 = d["y"].clone().requires_grad_(True)
         kk = rki.clone().requires_grad_(True)
-        yend = integrate(solver, y0, kk, 5.0, k)
+        yend = integrate(solver, y0, kk, 5.0, k, tape=tape)
         loss = (yend[io3] + 0.1 * yend[ino2]).sum()
         loss.backward()
         return y0.grad, kk.grad
@@ -218,13 +222,24 @@ def adjoint_leg(args, mech, solver, dev):
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
     print(f"[bench] adjoint leg: forward {t_fwd * 1e3:.0f} ms", file=sys.stderr, flush=True)
+    # the replay mode (only sync-step states kept, stages recomputed per sync
+    # step), then the stored tape (every accepted step kept, reverse sweeps only)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    gy, gk = fwd_adj()
+    fwd_adj("replay")
+    torch.cuda.synchronize()
+    t_replay = time.perf_counter() - t0
+    rep, rev, chunks = solver.last_adjoint_timing()
+    print(f"[bench] adjoint leg: forward+adjoint with replay {t_replay * 1e3:.0f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gy, gk = fwd_adj(args.adj_tape)
     torch.cuda.synchronize()
     t_both = time.perf_counter() - t0
-    rep, rev, chunks = solver.last_adjoint_timing()
-    print(f"[bench] adjoint leg: forward+adjoint {t_both * 1e3:.0f} ms", file=sys.stderr, flush=True)
+    rep_s, rev_s, chunks_s = solver.last_adjoint_timing()
+    stored = chunks_s == 0
+    print(f"[bench] adjoint leg: forward+adjoint with {'stored tape' if stored else 'replay'} {t_both * 1e3:.0f} ms",
+          file=sys.stderr, flush=True)
     bad = int((~torch.isfinite(gy)).any(dim=0).sum())
     # one sync step with one tangent-linear direction (d ln O3 of every cell
     # and d ln k of all reactions at once) against the plain solve
@@ -258,10 +273,15 @@ def adjoint_leg(args, mech, solver, dev):
         "workload": f"{mech['mechname'].lower()} ROS3, mixed regimes{' + VOC/SVOC' if args.adj_svoc else ''}, "
                     f"{n} cells, {k} sync steps of 5 min; "
                     f"loss = sum over cells of final O3 + 0.1 NO2; gradients w.r.t. all initial "
-                    f"concentrations and all rate constants; sync-step states stored, stages recomputed",
+                    f"concentrations and all rate constants; "
+                    + ("every accepted step stored, reverse sweeps only" if stored
+                       else "sync-step states stored, stages recomputed"),
         "ncells": n, "sync_steps": k, "variant": solver.variant, "fwd_ms": t_fwd * 1e3, "fwd_adj_ms": t_both * 1e3,
+        "tape": "stored" if stored else "replay",
         "accepted_cell_steps": accepted,
-        "last_sync_step_replay_ms": rep, "last_sync_step_reverse_ms": rev, "tape_chunks": chunks,
+        "last_sync_step_reverse_ms": rev_s,
+        "replay_mode": {"fwd_adj_ms": t_replay * 1e3, "fwd_adj_over_fwd": t_replay / t_fwd,
+                        "last_sync_step_replay_ms": rep, "last_sync_step_reverse_ms": rev, "tape_chunks": chunks},
         "cells_without_gradient": bad,
         "tlm_one_direction_over_fwd": t_tlm / t_one, "tlm_sync_step_ms": t_tlm * 1e3,
         "tlm_four_directions_over_fwd": t_tlm4 / t_one,

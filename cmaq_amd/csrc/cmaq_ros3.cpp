@@ -508,6 +508,7 @@ extern "C" int cmaq_ros3_query(const cmaq_ros3_ctx *c, const char *key, int64_t 
     else if (!std::strcmp(key, "grid_waves")) *value = c->waves;
     else if (!std::strcmp(key, "solve_waves")) *value = (int64_t)c->grid_blocks * (kSolveBlock / kWave);
     else if (!std::strcmp(key, "workspace_doubles_per_lane")) *value = k->nws;
+    else if (!std::strcmp(key, "tape_record_doubles")) *value = k->tape_rs;
     else return CMAQ_ROS3_E_ARG;
     return 0;
 }
@@ -807,6 +808,114 @@ extern "C" int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *c, int64_t n, double chem
         c->d_tape = nullptr;
         c->tape_bytes = 0;
     }
+    return check_fail(c, small, fail);
+}
+
+// Cells longest first into c->d_order (allocated on demand), from the
+// forward's accepted-step counts already copied to the host.
+static int upload_longest_first(cmaq_ros3_ctx *c, const std::vector<int32_t> &na, hipStream_t s)
+{
+    const int64_t n = (int64_t)na.size();
+    std::vector<int32_t> order((size_t)n);
+    for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return na[a] > na[b]; });
+    if (c->tape_off_cells < n) {
+        hipFree(c->d_tape_off);
+        hipFree(c->d_order);
+        c->d_tape_off = nullptr;
+        c->d_order = nullptr;
+        c->tape_off_cells = 0;
+        if (hipMalloc((void **)&c->d_tape_off, (size_t)n * sizeof(int64_t)) != hipSuccess ||
+            hipMalloc((void **)&c->d_order, (size_t)n * sizeof(int32_t)) != hipSuccess)
+            return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of tape offsets failed");
+        c->tape_off_cells = n;
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->d_order, order.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int cmaq_ros3_solve_tape_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,
+                                           double *y, int32_t *nacc, int32_t *nrej, double *tape,
+                                           int64_t tape_cap, uint64_t *tape_count, int64_t *tape_head,
+                                           cmaq_ros3_fail *fail, void *stream)
+{
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (n > 0 && (!rki || !y || !nacc || !tape || !tape_count || !tape_head || tape_cap < 0))
+        return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
+    HIP_TRY(c, hipEventRecord(c->ev[0], s));
+    HIP_TRY(c, hipEventRecord(c->ev[1], s));
+    SolveParams P;
+    solve_params(c, P, n, chemstep, rki, y, nacc, nrej);
+    if (c->user_order && c->user_order_n == n) P.order = c->user_order;
+    P.tape = tape;
+    P.tape_cap = tape_cap;
+    P.tape_count = reinterpret_cast<unsigned long long *>(tape_count);
+    P.tape_head = tape_head;
+    void *args[] = {&P};
+    HIP_TRY(c, hipLaunchKernel(c->mk->replay_kernel, dim3(c->grid_replay), dim3(kSolveBlock), args, 0, s));
+    HIP_TRY(c, hipEventRecord(c->ev[2], s));
+    unsigned char small[64];
+    uint64_t used = 0;
+    HIP_TRY(c, hipMemcpyAsync(small, c->d_small, 64, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&used, tape_count, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    float ms_s = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
+    c->rates_ms = 0.0;
+    c->solve_ms = ms_s;
+    const int rc = check_fail(c, small, fail);
+    if (used > (uint64_t)tape_cap)
+        return set_err(c, CMAQ_ROS3_E_NOMEM, "tape arena full: " + std::to_string(used) + " records taken, capacity " +
+                                                 std::to_string(tape_cap));
+    return rc;
+}
+
+extern "C" int cmaq_ros3_reverse_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,
+                                        const int32_t *nacc_fwd, const double *tape, int64_t tape_cap,
+                                        const uint64_t *tape_count, const int64_t *tape_head, double *lam,
+                                        double *grad_rki, cmaq_ros3_fail *fail, void *stream)
+{
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (n > 0 && (!rki || !nacc_fwd || !tape || !tape_count || !tape_head || !lam || tape_cap < 0))
+        return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<int32_t> na((size_t)n);
+    uint64_t used = 0;
+    HIP_TRY(c, hipMemcpyAsync(na.data(), nacc_fwd, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&used, tape_count, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    if (used > (uint64_t)tape_cap)
+        return set_err(c, CMAQ_ROS3_E_NOMEM, "the tape arena overflowed during the forward run");
+    if (int rc = upload_longest_first(c, na, s)) return rc;
+    SolveParams P;
+    solve_params(c, P, n, chemstep, rki, nullptr, nullptr, nullptr);
+    P.tape = const_cast<double *>(tape);
+    P.tape_cap = tape_cap;
+    P.tape_head = const_cast<int64_t *>(tape_head);
+    P.nacc_fwd = nacc_fwd;
+    P.lam = lam;
+    P.grad_rki = grad_rki;
+    P.order = c->d_order;
+    void *args[] = {&P};
+    HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
+    HIP_TRY(c, hipEventRecord(c->ev[1], s));
+    HIP_TRY(c, hipLaunchKernel(c->mk->reverse_kernel, dim3(c->grid_reverse), dim3(kSolveBlock), args, 0, s));
+    HIP_TRY(c, hipEventRecord(c->ev[2], s));
+    unsigned char small[64];
+    HIP_TRY(c, hipMemcpyAsync(small, c->d_small, 64, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    float ms_b = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms_b, c->ev[1], c->ev[2]));
+    c->replay_ms = 0.0;
+    c->reverse_ms = ms_b;
+    c->adj_chunks = 0;
     return check_fail(c, small, fail);
 }
 

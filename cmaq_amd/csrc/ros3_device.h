@@ -263,6 +263,11 @@ struct SolveParams {
     double *grad_rki;         // [nr][stride] accumulated dL/drki, or nullptr
     const double *dk;         // tangent-linear run: [ndir][nr][stride] rate directions, or nullptr
     int32_t ndir;             // tangent-linear run: directions (1..kTlmMaxDirections)
+    // stored tape (no replay): records appended to an arena of tape_cap
+    // records, each linked to its cell's previous record of the sync step
+    int64_t *tape_head;       // [stride] last record of each cell (-1: none), or nullptr
+    unsigned long long *tape_count;   // records taken from the arena so far
+    int64_t tape_cap;         // arena capacity in records
 };
 
 __device__ inline int lane_id() { return __lane_id(); }

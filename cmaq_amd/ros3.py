@@ -91,6 +91,10 @@ def lib():
         L.cmaq_ros3_solve_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 4 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_adjoint_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 5 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_last_adjoint_timing.argtypes = [vp, _f64p, _f64p, C.POINTER(C.c_int64)]
+        L.cmaq_ros3_solve_tape_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 5 + [C.c_int64] + \
+            [vp] * 2 + [C.POINTER(_Fail), vp]
+        L.cmaq_ros3_reverse_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 3 + [C.c_int64] + \
+            [vp] * 4 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_run_steps_device.argtypes = [vp, C.c_int64, C.c_double, C.c_int32] + [vp] * 10 + \
             [C.POINTER(_Fail), vp]
         L.cmaq_ros3_last_steps_timing.argtypes = [vp, _f64p]
@@ -489,6 +493,66 @@ class Ros3Solver:
                                             _ptr(lam), _ptr(grad_rki), C.byref(f), C.c_void_p(stream))
         if rc != 0:
             raise self._err(rc, {"code": rc, "cell": f.cell})
+
+    def new_tape(self, records, device=None):
+        """An arena for the stored-tape adjoint: (tape, count), with room for
+        `records` accepted steps and the record counter at zero."""
+        import torch
+        rs = self.query("tape_record_doubles")
+        tape = torch.empty(max(1, int(records)) * rs, dtype=torch.float64, device=device)
+        count = torch.zeros(1, dtype=torch.int64, device=device)
+        return tape, count
+
+    def solve_tape_device(self, chemstep, rki, y, nacc, tape, tape_count, tape_head, nrej=None, stream=None,
+                          raise_on_fail=True):
+        """solve_device that also appends every accepted step to the arena
+        (tape, tape_count) from new_tape; tape_head [n] int64 receives each
+        cell's last record.  Returns False, with y and nacc complete, if the
+        arena overflowed (its tape is then unusable), else True."""
+        import torch
+        n = y.numel() // self.ns
+        self._check([("rki", rki, self.nr), ("y", y, self.ns), ("nacc", nacc, 1), ("nrej", nrej, 1)], n)
+        self._check_tape(tape, tape_count, tape_head, n)
+        cap = tape.numel() // self.query("tape_record_doubles")
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_ros3_solve_tape_device(self._ctx, n, chemstep, _ptr(rki), _ptr(y), _ptr(nacc), _ptr(nrej),
+                                               _ptr(tape), cap, _ptr(tape_count), _ptr(tape_head), C.byref(f),
+                                               C.c_void_p(stream))
+        if rc == -6:          # the call allocates nothing: -6 is the arena overflow
+            return False
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, {"code": rc, "cell": f.cell})
+        return True
+
+    def reverse_device(self, chemstep, rki, nacc_fwd, tape, tape_count, tape_head, lam, grad_rki=None,
+                       stream=None):
+        """The adjoint of one solve_tape_device sync step from its stored
+        records (no replay): lam and grad_rki as in adjoint_device."""
+        import torch
+        n = lam.numel() // self.ns
+        self._check([("rki", rki, self.nr), ("nacc_fwd", nacc_fwd, 1), ("lam", lam, self.ns),
+                     ("grad_rki", grad_rki, self.nr)], n)
+        self._check_tape(tape, tape_count, tape_head, n)
+        cap = tape.numel() // self.query("tape_record_doubles")
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_ros3_reverse_device(self._ctx, n, chemstep, _ptr(rki), _ptr(nacc_fwd), _ptr(tape), cap,
+                                            _ptr(tape_count), _ptr(tape_head), _ptr(lam), _ptr(grad_rki),
+                                            C.byref(f), C.c_void_p(stream))
+        if rc != 0:
+            raise self._err(rc, {"code": rc, "cell": f.cell})
+
+    @staticmethod
+    def _check_tape(tape, tape_count, tape_head, n):
+        import torch
+        for name, t, dt, size in (("tape", tape, torch.float64, None), ("tape_count", tape_count, torch.int64, 1),
+                                  ("tape_head", tape_head, torch.int64, n)):
+            if not (t.is_cuda and t.is_contiguous() and t.dtype == dt) or (size is not None and t.numel() != size):
+                raise ValueError(f"{name} must be a contiguous CUDA {dt} tensor" +
+                                 (f" of {size} elements" if size is not None else ""))
 
     def last_adjoint_timing(self):
         """(replay_ms, reverse_ms, chunks) of the last adjoint_device call."""

@@ -191,6 +191,40 @@ int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep
                              double *grad_rki, cmaq_ros3_fail *fail, void *stream);
 
 /*
+ * Stored-tape adjoint, in two calls per sync step (no counterpart in the
+ * reference, as for cmaq_ros3_adjoint_device).
+ *
+ * cmaq_ros3_solve_tape_device integrates exactly as cmaq_ros3_solve_device
+ * (same y, n_accepted, n_rejected, failures) and also stores every accepted
+ * step (Y, K1, K2, K3, h) as one record of
+ * cmaq_ros3_query(ctx, "tape_record_doubles") doubles in the caller's arena
+ * `tape` of tape_cap records.  *tape_count (device, uint64) is the number of
+ * records taken so far: zero it before the first sync step; later calls
+ * append, so one arena holds a whole integration.  tape_head [ncells]
+ * (device, int64) receives each cell's last record of this sync step (-1 for
+ * a failed cell).  If the arena overflows, the integration is still complete
+ * but the tape is not: the call returns CMAQ_ROS3_E_NOMEM.
+ *
+ * cmaq_ros3_reverse_device is the reverse sweep of cmaq_ros3_adjoint_device
+ * over one such sync step, with no replay: lambda and grad_rki as there,
+ * n_accepted and tape_head from the matching cmaq_ros3_solve_tape_device
+ * call.  It returns CMAQ_ROS3_E_NOMEM without launching if *tape_count
+ * exceeds tape_cap.  Device memory, `stream` as in cmaq_ros3_run_device;
+ * both return after the stream has finished.
+ */
+int cmaq_ros3_solve_tape_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min,
+                                const double *rki, double *y_ppm, int32_t *n_accepted,
+                                int32_t *n_rejected, double *tape, int64_t tape_cap,
+                                uint64_t *tape_count, int64_t *tape_head,
+                                cmaq_ros3_fail *fail, void *stream);
+int cmaq_ros3_reverse_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min,
+                             const double *rki, const int32_t *n_accepted,
+                             const double *tape, int64_t tape_cap,
+                             const uint64_t *tape_count, const int64_t *tape_head,
+                             double *lambda, double *grad_rki, cmaq_ros3_fail *fail,
+                             void *stream);
+
+/*
  * Tangent-linear (forward-mode) sensitivities along ndir directions (1..8),
  * carried through a cmaq_ros3_solve_device call (no counterpart in the
  * reference; an exact alternative to DDM-3D's Crank-Nicolson sensitivity

@@ -156,6 +156,60 @@ def test_adjoint_in_chunks_equals_single_pass(cb6r5, solver, monkeypatch):
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
+def test_solve_tape_device_integrates_as_solve_device(cb6r5, solver):
+    """The recording forward gives the plain forward's y, nacc and nrej bit
+    for bit, takes one record per accepted step (failed cells too, up to
+    their failure), and links each cell's records into a chain of nacc
+    records ending at its head."""
+    import torch
+    n = 777
+    d = make_cells(cb6r5, np.arange(n), seed=26, workload="mixed")
+    rki = _rates(solver, d)
+    y1, y2 = _dev(d["y"]).clone(), _dev(d["y"]).clone()
+    a1, a2, r1, r2 = (torch.zeros(n, dtype=torch.int32, device="cuda") for _ in range(4))
+    solver.solve_device(5.0, rki, y1, a1, r1, raise_on_fail=False)
+    tape, count = solver.new_tape(n * 64, device="cuda")
+    head = torch.empty(n, dtype=torch.int64, device="cuda")
+    assert solver.solve_tape_device(5.0, rki, y2, a2, tape, count, head, nrej=r2, raise_on_fail=False)
+    assert torch.equal(y1, y2) and torch.equal(a1, a2) and torch.equal(r1, r2)
+    na, hd = a2.cpu().numpy(), head.cpu().numpy()
+    ok = int(na[na > 0].sum())
+    assert ok <= int(count.item()) <= ok + int((na < 0).sum()) * 1000
+    rs = solver.query("tape_record_doubles")
+    links = tape.view(-1, rs)[:, rs - 1].cpu().numpy()
+    for c in np.flatnonzero(na > 0)[::37]:
+        r, k = hd[c], 0
+        while r >= 0:
+            r, k = int(links[r]), k + 1
+        assert k == na[c], (c, k, na[c])
+    assert (hd[na < 0] == -1).all()
+
+
+def test_stored_tape_equals_replay(cb6r5, solver):
+    """Gradients through autograd over 3 sync steps: the stored tape (reverse
+    sweeps only), the replay path and a stored tape whose arena overflows
+    (falls back to the replay) agree bit for bit."""
+    import torch
+    from cmaq_amd.adjoint import integrate
+    n = 600
+    d = make_cells(cb6r5, np.arange(n), seed=27, workload="mixed")
+    rki = _rates(solver, d)
+    names = cb6r5["chemistry_spc"]
+    outs = []
+    for tape, frac in (("stored", 0.7), ("replay", 0.7), ("stored", 1e-12)):
+        y0 = _dev(d["y"]).requires_grad_(True)
+        kk = rki.clone().requires_grad_(True)
+        yend = integrate(solver, y0, kk, 5.0, 3, tape=tape, tape_fraction=frac)
+        (yend[names.index("O3")] + 0.1 * yend[names.index("NO2")]).sum().backward()
+        rep, _, chunks = solver.last_adjoint_timing()
+        outs.append((yend.detach().cpu().numpy(), y0.grad.cpu().numpy(), kk.grad.cpu().numpy(), rep, chunks))
+    assert outs[0][3] == 0.0 and outs[0][4] == 0        # no replay launched
+    assert outs[1][4] >= 1 and outs[2][4] >= 1           # replay launched
+    for o in outs[1:]:
+        for a, b in zip(outs[0][:3], o[:3]):
+            np.testing.assert_array_equal(a, b)
+
+
 def test_adjoint_vs_ddm3d_night_cells(cb6r5, oracle_cb6r5, solver):
     """GPU adjoint against the oracle's restatement of the CMAQ-DDM-3D
     chemistry step, contracted with the same seed vector, on night cells:
