@@ -31,7 +31,7 @@ from .ros3 import Ros3Solver
 
 class Ros3Integrate(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, y0, rki, solver, chemstep, nsteps, tape="stored", tape_fraction=0.7):
+    def forward(ctx, y0, rki, solver, chemstep, nsteps, tape="stored", tape_fraction=0.7, tape_records=None):
         if not isinstance(solver, Ros3Solver):
             raise TypeError("solver must be a Ros3Solver")
         if tape not in ("stored", "replay"):
@@ -44,7 +44,11 @@ class Ros3Integrate(torch.autograd.Function):
             free, _ = torch.cuda.mem_get_info(y.device)
             rs = solver.query("tape_record_doubles")
             cap = min(int(free * tape_fraction) // (8 * rs), n * nsteps * 64)
-            arena = solver.new_tape(cap, device=y.device)
+            if tape_records is not None:
+                cap = min(cap, int(tape_records))
+            # skip the attempt when even the fewest possible steps cannot fit
+            if n * nsteps * min_accepted_steps(chemstep) <= cap:
+                arena = solver.new_tape(cap, device=y.device)
         states, counts, heads = [], [], []
         for _ in range(nsteps):
             states.append(y.clone())
@@ -97,7 +101,23 @@ class Ros3Integrate(torch.autograd.Function):
             ctx.arena, ctx.heads = None, []
             del arena, heads
             _release(big)
-        return (lam if ctx.needs_input_grad[0] else None), grad_rki, None, None, None, None, None
+        return (lam if ctx.needs_input_grad[0] else None), grad_rki, None, None, None, None, None, None
+
+
+def min_accepted_steps(chemstep, dtstart=0.05, dtmax=10.0, facmax=10.0):
+    """Fewest accepted steps any cell can take in one sync step: the step
+    starts at DTSTART, may not grow on the first accepted step and grows at
+    most FACMAX-fold after that, up to DTMAX, and the last step ends exactly
+    at the sync step (rbsolver.F:104-113, 364-369, 509-515).  Four for a
+    5-minute sync step (0.05, 0.05, 0.5, 4.4)."""
+    t, h, k, first = 0.0, dtstart, 0, True
+    while t < chemstep:
+        h = min(h, chemstep - t)
+        t += h
+        k += 1
+        h = min(h * (1.0 if first else facmax), dtmax)
+        first = False
+    return k
 
 
 def _release(big):
@@ -106,9 +126,12 @@ def _release(big):
         torch.cuda.empty_cache()
 
 
-def integrate(solver, y0, rki, chemstep=5.0, nsteps=1, tape="stored", tape_fraction=0.7):
+def integrate(solver, y0, rki, chemstep=5.0, nsteps=1, tape="stored", tape_fraction=0.7, tape_records=None):
     """y_end after nsteps sync steps; differentiable in y0 and rki (reverse
     and forward mode).  tape: "stored" keeps every accepted step for the
-    backward pass (up to tape_fraction of the free device memory, else it
-    falls back to replay), "replay" recomputes the stages per sync step."""
-    return Ros3Integrate.apply(y0, rki, solver, chemstep, nsteps, tape, tape_fraction)
+    backward pass in an arena of up to tape_fraction of the free device
+    memory (and at most tape_records records, if given), else it falls back
+    to replay; "replay" recomputes the stages per sync step.  The stored
+    tape is not attempted when ncells x nsteps x min_accepted_steps(chemstep)
+    records cannot fit."""
+    return Ros3Integrate.apply(y0, rki, solver, chemstep, nsteps, tape, tape_fraction, tape_records)

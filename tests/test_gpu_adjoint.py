@@ -185,10 +185,33 @@ def test_solve_tape_device_integrates_as_solve_device(cb6r5, solver):
     assert (hd[na < 0] == -1).all()
 
 
+def test_solve_tape_device_overflow(cb6r5, solver):
+    """An arena too small for the sync step: the integration is still the
+    plain forward's, the call reports the overflow, and the reverse sweep
+    refuses the incomplete tape."""
+    import torch
+    from cmaq_amd.ros3 import Ros3Error
+    n = 300
+    d = make_cells(cb6r5, np.arange(n), seed=28, workload="mixed")
+    rki = _rates(solver, d)
+    y1, y2 = _dev(d["y"]).clone(), _dev(d["y"]).clone()
+    a1, a2 = (torch.zeros(n, dtype=torch.int32, device="cuda") for _ in range(2))
+    solver.solve_device(5.0, rki, y1, a1, raise_on_fail=False)
+    tape, count = solver.new_tape(10, device="cuda")
+    head = torch.empty(n, dtype=torch.int64, device="cuda")
+    assert not solver.solve_tape_device(5.0, rki, y2, a2, tape, count, head, raise_on_fail=False)
+    assert torch.equal(y1, y2) and torch.equal(a1, a2)
+    assert int(count.item()) > 10
+    lam = _dev(_seed(cb6r5, n)).clone()
+    with pytest.raises(Ros3Error):
+        solver.reverse_device(5.0, rki, a2, tape, count, head, lam)
+
+
 def test_stored_tape_equals_replay(cb6r5, solver):
     """Gradients through autograd over 3 sync steps: the stored tape (reverse
-    sweeps only), the replay path and a stored tape whose arena overflows
-    (falls back to the replay) agree bit for bit."""
+    sweeps only), the replay path, a stored tape whose arena overflows in the
+    first sync step (falls back to the replay) and one too small to attempt
+    agree bit for bit."""
     import torch
     from cmaq_amd.adjoint import integrate
     n = 600
@@ -196,15 +219,16 @@ def test_stored_tape_equals_replay(cb6r5, solver):
     rki = _rates(solver, d)
     names = cb6r5["chemistry_spc"]
     outs = []
-    for tape, frac in (("stored", 0.7), ("replay", 0.7), ("stored", 1e-12)):
+    for tape, frac, recs in (("stored", 0.7, None), ("replay", 0.7, None), ("stored", 0.7, n * 3 * 4),
+                             ("stored", 1e-12, None)):
         y0 = _dev(d["y"]).requires_grad_(True)
         kk = rki.clone().requires_grad_(True)
-        yend = integrate(solver, y0, kk, 5.0, 3, tape=tape, tape_fraction=frac)
+        yend = integrate(solver, y0, kk, 5.0, 3, tape=tape, tape_fraction=frac, tape_records=recs)
         (yend[names.index("O3")] + 0.1 * yend[names.index("NO2")]).sum().backward()
         rep, _, chunks = solver.last_adjoint_timing()
         outs.append((yend.detach().cpu().numpy(), y0.grad.cpu().numpy(), kk.grad.cpu().numpy(), rep, chunks))
     assert outs[0][3] == 0.0 and outs[0][4] == 0        # no replay launched
-    assert outs[1][4] >= 1 and outs[2][4] >= 1           # replay launched
+    assert all(o[4] >= 1 for o in outs[1:])              # replay launched
     for o in outs[1:]:
         for a, b in zip(outs[0][:3], o[:3]):
             np.testing.assert_array_equal(a, b)

@@ -118,3 +118,22 @@ def test_adjoint_run_matches_cellwise(cb6r5, oracle_cb6r5):
         assert r["nacc"][c] == len(h)
         assert np.array_equal(r["y"][:, c], yend)
         assert np.array_equal(r["lam"][:, c], l0) and np.array_equal(r["gk"][:, c], g)
+
+
+@pytest.mark.parametrize("chemstep", [5.0, 1.0, 0.05, 60.0])
+def test_min_accepted_steps_is_the_zero_rate_sequence(cb6r5, oracle_cb6r5, chemstep):
+    """The stored tape's lower bound on accepted steps per sync step is the
+    step sequence of a cell with no chemistry (zero error, so every step
+    grows as far as the controller allows): the oracle takes exactly that
+    many steps, and a cell with chemistry never takes fewer."""
+    from cmaq_amd.adjoint import min_accepted_steps
+    m = min_accepted_steps(chemstep)
+    d = make_cells(cb6r5, np.arange(8), seed=31, workload="mixed")
+    zero = np.zeros(cb6r5["nrxns"])
+    h, _ = oracle_cb6r5.steps_cell(chemstep, zero, d["y"][:, 0])
+    assert len(h) == m, (chemstep, h)
+    assert abs(h.sum() - chemstep) <= 1e-12 * chemstep
+    rk, _ = oracle_cb6r5.rates(d["temp"], d["pres"], d["h2o"], d["rj"])
+    for c in range(8):
+        hc, _ = oracle_cb6r5.steps_cell(chemstep, rk[:, c], d["y"][:, c])
+        assert len(hc) >= m
