@@ -223,7 +223,11 @@ def adjoint_leg(args, mech, solver, dev):
     t_fwd = time.perf_counter() - t0
     print(f"[bench] adjoint leg: forward {t_fwd * 1e3:.0f} ms", file=sys.stderr, flush=True)
     # the replay mode (only sync-step states kept, stages recomputed per sync
-    # step), then the stored tape (every accepted step kept, reverse sweeps only)
+    # step), then the stored tape (every accepted step kept, reverse sweeps
+    # only).  Each mode releases its large tape when it returns; a large
+    # allocation right after such a release can wait seconds for the device
+    # to reclaim the memory.  In this order the CB6r5 workload shows no such
+    # wait; the reverse order adds it to the replay mode.
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     fwd_adj("replay")
