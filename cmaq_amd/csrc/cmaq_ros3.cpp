@@ -1051,7 +1051,6 @@ extern "C" int cmaq_ros3_run_steps_device(cmaq_ros3_ctx *c, int64_t n, double ch
     }
     if (c->step_cells < n) {
         hipFree(c->d_step_na); hipFree(c->d_step_nr);
-    hipFree(c->d_map); hipFree(c->d_block); hipFree(c->d_grid);
         c->d_step_na = c->d_step_nr = nullptr;
         c->step_cells = 0;
         if (hipMalloc((void **)&c->d_step_na, (size_t)n * 4) != hipSuccess ||

@@ -116,3 +116,45 @@ def test_map_errors(setup):
     assert e.value.code == -1
     with pytest.raises(Ros3Error):
         fresh.set_cgrid_map(int(idx.max()) - 1, idx, conv, mw)
+
+
+def test_chem_after_steps_loop_on_one_context(setup):
+    """One context used as bench.py uses it: the CGRID map is set, the graph
+    sync loop runs on a cell count below the grid's (which once released the
+    species map and the CHEM buffers), then a CHEM call.  The CHEM result and
+    the loop's must equal those of fresh contexts bit for bit."""
+    import torch
+    from cmaq_amd.inputs import make_cells
+    from cmaq_amd.ros3 import Ros3Solver
+    name, m, o, s, g, idx, conv, mw = setup
+    ctx = Ros3Solver(m, RTOL, ATOL)
+    ctx.set_cgrid_map(g["nspcsd"], idx, conv, mw)
+    n = 777
+    d = make_cells(m, np.arange(n), seed=11)
+    t = {k: torch.as_tensor(d[k], device="cuda") for k in ("temp", "pres", "h2o", "rj", "y")}
+
+    def loop(solver):
+        y = t["y"].clone()
+        na = torch.zeros(2, n, dtype=torch.int32, device="cuda")
+        solver.run_steps_device(5.0, 2, t["temp"], t["pres"], t["h2o"], t["rj"], y, nacc=na,
+                                raise_on_fail=False)
+        torch.cuda.synchronize()
+        return y.cpu().numpy(), na.cpu().numpy()
+
+    def chem(solver):
+        dg = _dev_grid(g)
+        na = torch.zeros(g["ta"].size, dtype=torch.int32, device="cuda")
+        solver.chem_device(dg, 5.0, nacc=na, raise_on_fail=False)
+        return dg["cgrid"].cpu().numpy(), na.cpu().numpy()
+
+    y1, na1 = loop(ctx)
+    c1, nc1 = chem(ctx)
+    y2, na2 = loop(ctx)
+    fresh = Ros3Solver(m, RTOL, ATOL)
+    fresh.set_cgrid_map(g["nspcsd"], idx, conv, mw)
+    c0, nc0 = chem(fresh)
+    y0, na0 = loop(Ros3Solver(m, RTOL, ATOL))
+    ctx.close()
+    assert np.array_equal(c1, c0) and np.array_equal(nc1, nc0), name
+    assert np.array_equal(y1, y0) and np.array_equal(na1, na0), name
+    assert np.array_equal(y2, y0) and np.array_equal(na2, na0), name
