@@ -30,21 +30,26 @@ MECHS = {n: n + ".json" for n in (
 # the GPU).  The table keeps the winners; tuning.json records the others that
 # were measured.
 VARIANTS = {
-    "cb6r5_ae7_aq": [(170, 96, "tile10x10+k1"), (110, 32, "column+k1")],
-    "cracmm2": [(170, 96, "tile12x12+k1")],
-    "cb6r3_ae7_aq": [(170, 96, "tile10x10+k1")],
-    "cb6r5m_ae7_aq": [(150, 64, "column+k1")],
-    "racm2_ae6_aq": [(170, 96, "tile12x12+k1")],
-    "cracmm1_aq": [(170, 96, "tile12x12+k1")],
-    "cracmm1amore_aq": [(170, 96, "tile12x12+k1")],
-    "saprc07tc_ae6_aq": [(170, 96, "tile12x12+k1")],
-    "saprc07tic_ae7i_aq": [(170, 96, "tile20x20+k1")],
+    "cb6r5_ae7_aq": [(170, 96, "ctile10x10+k1"), (170, 96, "tile10x10+k1"), (170, 96, "ctile14x10+k1")],
+    "cracmm2": [(170, 96, "ctile12x10+k1"), (170, 96, "tile12x12+k1")],
+    "cb6r3_ae7_aq": [(170, 96, "ctile14x14+k1")],
+    "cb6r5m_ae7_aq": [(170, 96, "ctile8x8+k1")],
+    "racm2_ae6_aq": [(170, 96, "ctile14x10+k1")],
+    "cracmm1_aq": [(170, 96, "ctile14x10+k1")],
+    "cracmm1amore_aq": [(170, 96, "ctile14x14+k1")],
+    "saprc07tc_ae6_aq": [(170, 96, "ctile8x8+k1")],
+    "saprc07tic_ae7i_aq": [(170, 96, "ctile10x10+k1")],
 }
 
 
 def variant_name(b, l, order):
     base, *extra = order.split("+")
-    suffix = "" if base == "column" else "_t" + base[4:]
+    if base == "column":
+        suffix = ""
+    elif base.startswith("ctile"):
+        suffix = "_ct" + base[5:]
+    else:
+        suffix = "_t" + base[4:]
     for e in extra:
         suffix += "_" + e
     return f"b{b}_l{l}{suffix}"
@@ -86,7 +91,11 @@ def tuned(name):
         with open(TUNING) as f:
             table = json.load(f)
     entry = table.get(name, {})
-    return entry.get("variant", variant_name(*VARIANTS[name][0])), int(entry.get("refill_min", 2))
+    built = [variant_name(*v) for v in VARIANTS[name]]
+    var = entry.get("variant")
+    if var not in built:       # a variant no longer generated: the table's first
+        var = built[0]
+    return var, int(entry.get("refill_min", 1))
 
 
 def generate(names=None):

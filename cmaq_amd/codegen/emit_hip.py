@@ -179,6 +179,9 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     # "+step": the whole attempted step (f(Y), Jacobian + LU, three stage
     # solves, f(YP)) as one function, so registers carry values across the
     # phases and the callee-saved registers are saved once per step
+    # the factored pivots: the solves read their reciprocals VD, never the
+    # pivots themselves, so the factorization does not write them back
+    piv = frozenset(("LU", day.diag[j]) for j in range(ns))
     base_order = jac_order.split("+")[0]
     flags = jac_order.split("+")[1:]
     merge = "step" in flags
@@ -217,7 +220,8 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
         body += fn("feval", ", int ys_off_", items, off, budget, {"YS": "ys_off_"})
         b("")
         ops = jac_lu_ops(sym, base_order, fuse_k1)
-        items, cnt = allocate(ops, {"LU", "VD", "K1"} if fuse_k1 else {"LU", "VD"}, budget, lookahead, entry_all, lds)
+        items, cnt = allocate(ops, {"LU", "VD", "K1"} if fuse_k1 else {"LU", "VD"}, budget, lookahead, entry_all, lds,
+                              dead=piv)
         stats["jac_lu"] = _stat(ops, cnt)
         jac_lu_text = fn("jac_lu", ", double gdtinv", items, off, budget)
         body += jac_lu_text
@@ -236,7 +240,7 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     if fuse_k1:
         # the adjoint's refactorization has no f(Y) to substitute
         ops = jac_lu_ops(sym, base_order)
-        items, cnt = allocate(ops, {"LU", "VD"}, budget, lookahead, entry_all)
+        items, cnt = allocate(ops, {"LU", "VD"}, budget, lookahead, entry_all, dead=piv)
         body_adj += fn("jac_lu_plain", ", double gdtinv", items, off, budget)
         body_adj.append("")
     elif not merge:

@@ -182,14 +182,18 @@ def _jac_lu_ops(sym, order="column"):
             for s in day.l_scale[j]:
                 ops.append(Op("{0} = {0} * {1};", [("LU", s), ("VD", j)], True))
         return ops
-    if order.startswith("tile"):
-        # 2-D tiles of br x bc entries, tiles row-block by row-block: a tile's
-        # entries take their updates k by k (each L(i,k) and U(k,j) loaded once
-        # per tile), a column's pivot and L scaling come when k reaches it.
-        # Every operand is final when used: L(i,k) lies in an earlier tile of
-        # the same row block (or this one), U(k,j) in an earlier row block of
-        # the same column block (or this tile).
-        br, bc = (int(v) for v in order[4:].split("x"))
+    if order.startswith("tile") or order.startswith("ctile"):
+        # 2-D tiles of br x bc entries, "tile": row block by row block,
+        # "ctile": column block by column block (top to bottom within a column
+        # block, so the Jacobian terms of a column's reactions and the pivot
+        # rows' U entries stay close).  A tile's entries take their updates k
+        # by k (each L(i,k) and U(k,j) loaded once per tile), a column's pivot
+        # and L scaling come when k reaches it.  Every operand is final when
+        # used in either order: L(i,k) lies in an earlier tile of the same row
+        # block (or this one), U(k,j) in an earlier row block of the same
+        # column block (or this tile).
+        colmajor = order.startswith("ctile")
+        br, bc = (int(v) for v in order[5 if colmajor else 4:].split("x"))
         upd = {}
         for (ij, ik, kj) in day.lu_ops:
             upd.setdefault(ij, []).append((day.cols[ik], ik, kj))
@@ -208,7 +212,7 @@ def _jac_lu_ops(sym, order="column"):
                     out.append(Op("{0} = {0} * {1};", [("LU", s), ("VD", k)], True))
             return out
 
-        for key in sorted(tiles):
+        for key in sorted(tiles, key=(lambda k: (k[1], k[0])) if colmajor else None):
             ents = sorted(tiles[key], key=lambda s: (day.rows[s], day.cols[s]))
             in_tile = set(ents)
             for s in ents:

@@ -48,10 +48,11 @@ def _reads(op):
             if isinstance(a, tuple) and (k > 0 or op.rmw)]
 
 
-def allocate(ops, outputs, budget, lookahead, defined_on_entry, lds_slots=0):
+def allocate(ops, outputs, budget, lookahead, defined_on_entry, lds_slots=0, dead=frozenset()):
     """Returns (items, counts) with counts of each item kind.
 
-    outputs: array names whose final values must be in memory at the end.
+    outputs: array names whose final values must be in memory at the end,
+    except for the slots in `dead` (values no later function reads).
     defined_on_entry(slot): True if memory holds the slot's value on entry."""
     R, S = budget, lds_slots
     n = len(ops)
@@ -84,7 +85,7 @@ def allocate(ops, outputs, budget, lookahead, defined_on_entry, lds_slots=0):
 
     def needs_store(r, t):
         s = slot_of[r]
-        return dirty[r] and (next_use(s, t) is not None or s[0] in outputs)
+        return dirty[r] and (next_use(s, t) is not None or (s[0] in outputs and s not in dead))
 
     def lds_put(r, t):
         """Keep a copy of register r's (clean) value in LDS if it is worth it."""
@@ -204,6 +205,6 @@ def allocate(ops, outputs, budget, lookahead, defined_on_entry, lds_slots=0):
             pinned[r] = max(pinned[r], t)
     for r in range(R):
         s = slot_of[r]
-        if s is not None and dirty[r] and s[0] in outputs:
+        if s is not None and dirty[r] and s[0] in outputs and s not in dead:
             emit("store", r, s)
     return items, dict(counts)

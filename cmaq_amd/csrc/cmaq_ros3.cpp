@@ -60,7 +60,15 @@ struct cmaq_ros3_ctx {
     int32_t *d_fail_flag = nullptr, *d_fail_code = nullptr;
     int64_t *d_fail_cell = nullptr;
     double *d_fail_info = nullptr;
+    double *d_fail_detail = nullptr;   // ERR, then Y, YP, K1, K2, K3 [ns] of the first failing cell
     void *d_small = nullptr;
+    // host copy of the last call's first failure (cmaq_ros3_fail_detail)
+    int32_t fd_code = 0;
+    int64_t fd_cell = -1;
+    double fd_info[3] = {0.0, 0.0, 0.0};           // maxerr, dt, tnow
+    std::vector<double> fd_raw;                    // 1 + 5 * ns doubles, solver order
+    double fd_met[5] = {NAN, NAN, NAN, NAN, NAN};  // temp, pres, h2o, seawater, dens
+    std::vector<double> fd_rj, fd_rki;
     double *d_rki = nullptr;
     int64_t rki_cells = 0;
     // adjoint tapes
@@ -450,6 +458,7 @@ extern "C" int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech, c
         if (dalloc(c, &c->d_mul, std::max<size_t>(c->mul.size(), 1))) goto fail_nomem;
         if (dalloc(c, &c->d_rtol, c->ns) || dalloc(c, &c->d_atol, c->ns)) goto fail_nomem;
         if (dalloc(c, (char **)&c->d_small, 256)) goto fail_nomem;
+        if (dalloc(c, &c->d_fail_detail, 1 + 5 * (size_t)c->ns)) goto fail_nomem;
         c->d_queue = (unsigned long long *)c->d_small;
         c->d_fail_cell = (int64_t *)((char *)c->d_small + 8);
         c->d_fail_info = (double *)((char *)c->d_small + 16);
@@ -477,7 +486,7 @@ extern "C" void cmaq_ros3_free(cmaq_ros3_ctx *c)
     if (!c) return;
     hipSetDevice(c->device);
     hipFree(c->d_ops); hipFree(c->d_mul); hipFree(c->d_rtol); hipFree(c->d_atol);
-    hipFree(c->d_ws); hipFree(c->d_small); hipFree(c->d_rki); hipFree(c->d_io);
+    hipFree(c->d_ws); hipFree(c->d_small); hipFree(c->d_fail_detail); hipFree(c->d_rki); hipFree(c->d_io);
     hipFree(c->d_tape); hipFree(c->d_tape_off); hipFree(c->d_order);
     hipFree(c->d_step_na); hipFree(c->d_step_nr);
     hipFree(c->d_map); hipFree(c->d_block); hipFree(c->d_grid);
@@ -546,6 +555,48 @@ extern "C" int cmaq_ros3_species_order(const cmaq_ros3_ctx *c, int32_t *inew2old
     return 0;
 }
 
+extern "C" int cmaq_ros3_get_fail_detail(const cmaq_ros3_ctx *c, cmaq_ros3_fail_detail *d, double *spc_err,
+                                     double *y, double *yp, double *rj, double *rki)
+{
+    if (!c || !d) return CMAQ_ROS3_E_ARG;
+    std::memset(d, 0, sizeof *d);
+    d->numb_mech_spc = c->ns;
+    d->nrxns = c->nr;
+    d->nphotab = c->nphotab;
+    d->cell = -1;
+    d->temp_k = d->pres_atm = d->h2o_ppm = d->seawater = d->dens = NAN;
+    d->maxerr = d->err = d->dt = d->tnow = NAN;
+    const int ns = c->ns;
+    auto fill = [](double *a, int n) { if (a) for (int i = 0; i < n; ++i) a[i] = NAN; };
+    fill(spc_err, ns); fill(y, ns); fill(yp, ns); fill(rj, c->nphotab); fill(rki, c->nr);
+    if (c->fd_code == 0) return 0;
+    d->code = c->fd_code;
+    d->cell = c->fd_cell;
+    d->maxerr = c->fd_info[0];
+    d->dt = c->fd_info[1];
+    d->tnow = c->fd_info[2];
+    d->temp_k = c->fd_met[0]; d->pres_atm = c->fd_met[1]; d->h2o_ppm = c->fd_met[2];
+    d->seawater = c->fd_met[3]; d->dens = c->fd_met[4];
+    if (c->fd_raw.size() != 1 + 5 * (size_t)ns) return 0;
+    const double *r = c->fd_raw.data();
+    const double err = r[0];
+    d->err = err;
+    for (int i = 0; i < ns; ++i) {          // solver order -> mechanism order
+        const int old = c->mk->inew2old[i];
+        const double vy = r[1 + i], vyp = r[1 + ns + i];
+        const double k1 = r[1 + 2 * ns + i], k2 = r[1 + 3 * ns + i], k3 = r[1 + 4 * ns + i];
+        // MAX_SPC_ERR of rbsolver.F:551-555
+        const double ytol = c->atol_new[i] + c->rtol_new[i] * std::fabs(vyp);
+        const double e = (D1 * k1 + D2 * k2 + D3 * k3) / ytol;
+        if (spc_err) spc_err[old] = e * e / err;
+        if (y) y[old] = vy;
+        if (yp) yp[old] = vyp;
+    }
+    if (rj) for (size_t k = 0; k < c->fd_rj.size(); ++k) rj[k] = c->fd_rj[k];
+    if (rki) for (size_t k = 0; k < c->fd_rki.size(); ++k) rki[k] = c->fd_rki[k];
+    return 0;
+}
+
 static int launch_rates(cmaq_ros3_ctx *c, int64_t n, const double *temp, const double *pres,
                         const double *h2o, const double *rj, const double *het, const double *sea,
                         double *rki, hipStream_t s)
@@ -591,6 +642,7 @@ static void solve_params(cmaq_ros3_ctx *c, SolveParams &P, int64_t n, double che
     P.rtol = c->d_rtol; P.atol = c->d_atol; P.ws = c->d_ws; P.ws_stride = c->ws_stride; P.queue = c->d_queue;
     P.fail_flag = c->d_fail_flag; P.fail_cell = c->d_fail_cell; P.fail_info = c->d_fail_info;
     P.fail_code = c->d_fail_code; P.max_attempts = c->max_attempts;
+    P.fail_detail = c->d_fail_detail;
     P.refill_min = c->refill_min > 0 ? c->refill_min : (c->mk->refill_min > 0 ? c->mk->refill_min : 2);
 }
 
@@ -616,6 +668,15 @@ static int check_fail(cmaq_ros3_ctx *c, const unsigned char *small, cmaq_ros3_fa
     std::memcpy(&code, small + 44, 4);
     if (!flag) return 0;
     if (fail) { fail->code = code; fail->cell = cell; fail->maxerr = info[0]; fail->dt = info[1]; fail->tnow = info[2]; }
+    if (c->fd_code == 0) {   // keep the first failure of a call (the streams have finished)
+        c->fd_code = code;
+        c->fd_cell = cell;
+        std::memcpy(c->fd_info, info, sizeof info);
+        c->fd_raw.assign(1 + 5 * (size_t)c->ns, NAN);
+        if (hipMemcpy(c->fd_raw.data(), c->d_fail_detail, c->fd_raw.size() * sizeof(double),
+                      hipMemcpyDeviceToHost) != hipSuccess)
+            c->fd_raw.assign(c->fd_raw.size(), NAN);
+    }
     char buf[200];
     std::snprintf(buf, sizeof buf, "%s in cell %lld (MAXERR %.4e, DT %.4e min, TNOW %.4e min)", fail_text(code),
                   (long long)cell, info[0], info[1], info[2]);
@@ -653,10 +714,53 @@ static int launch_solve(cmaq_ros3_ctx *c, int64_t n, double chemstep, const doub
     return check_fail(c, small, fail);
 }
 
+static void clear_fail_detail(cmaq_ros3_ctx *c)
+{
+    c->fd_code = 0;
+    c->fd_cell = -1;
+    c->fd_raw.clear();
+    for (double &v : c->fd_met) v = NAN;
+    c->fd_rj.clear();
+    c->fd_rki.clear();
+}
+
+// One element of a device array (cell-fastest [rows][n]) for the failure
+// record; NaN if the array is absent or the copy fails.
+template <class T>
+static double fetch_cell(const T *a, int64_t n, int64_t row, int64_t cell)
+{
+    T v;
+    if (!a || cell < 0 || cell >= n) return NAN;
+    if (hipMemcpy(&v, a + row * n + cell, sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) return NAN;
+    return (double)v;
+}
+
+// The failing cell's inputs, as rbsolver.F:563-580 writes them (BLKTEMP,
+// BLKPRES, BLKDENS, BLKCH2O, BLKSEAWATER, RJBLK, RKI); device arrays of the call.
+static void note_fail_inputs(cmaq_ros3_ctx *c, int64_t n, const double *temp, const double *pres,
+                             const double *h2o, const double *rj, const double *sea, const double *rki)
+{
+    if (c->fd_code == 0) return;
+    const int64_t cell = c->fd_cell;
+    if (temp) c->fd_met[0] = fetch_cell(temp, n, 0, cell);
+    if (pres) c->fd_met[1] = fetch_cell(pres, n, 0, cell);
+    if (h2o) c->fd_met[2] = fetch_cell(h2o, n, 0, cell);
+    if (temp) c->fd_met[3] = sea ? fetch_cell(sea, n, 0, cell) : 0.0;
+    if (rj) {
+        c->fd_rj.resize(c->nphotab);
+        for (int k = 0; k < c->nphotab; ++k) c->fd_rj[k] = fetch_cell(rj, n, k, cell);
+    }
+    if (rki) {
+        c->fd_rki.resize(c->nr);
+        for (int k = 0; k < c->nr; ++k) c->fd_rki[k] = fetch_cell(rki, n, k, cell);
+    }
+}
+
 static int check_run_args(cmaq_ros3_ctx *c, int64_t n, double chemstep, cmaq_ros3_fail *fail)
 {
     if (fail) { fail->code = 0; fail->pad = 0; fail->cell = -1; fail->maxerr = fail->dt = fail->tnow = 0.0; }
     if (!c) return CMAQ_ROS3_E_ARG;
+    clear_fail_detail(c);
     if (n < 0 || !(chemstep > 0.0)) return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
     return 0;
 }
@@ -684,7 +788,9 @@ extern "C" int cmaq_ros3_run_device(cmaq_ros3_ctx *c, int64_t n, double chemstep
     HIP_TRY(c, hipEventRecord(c->ev[0], s));
     int rc = launch_rates(c, n, temp, pres, h2o, rj, het, sea, c->d_rki, s);
     if (rc) return rc;
-    return launch_solve(c, n, chemstep, c->d_rki, y, nacc, nrej, fail, s);
+    rc = launch_solve(c, n, chemstep, c->d_rki, y, nacc, nrej, fail, s);
+    note_fail_inputs(c, n, temp, pres, h2o, rj, sea, c->d_rki);
+    return rc;
 }
 
 extern "C" int cmaq_ros3_solve_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,
@@ -698,7 +804,9 @@ extern "C" int cmaq_ros3_solve_device(cmaq_ros3_ctx *c, int64_t n, double chemst
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
     HIP_TRY(c, hipEventRecord(c->ev[0], s));
-    return launch_solve(c, n, chemstep, rki, y, nacc, nrej, fail, s);
+    const int rc = launch_solve(c, n, chemstep, rki, y, nacc, nrej, fail, s);
+    note_fail_inputs(c, n, nullptr, nullptr, nullptr, nullptr, nullptr, rki);
+    return rc;
 }
 
 extern "C" int cmaq_ros3_tlm_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,
@@ -1107,7 +1215,9 @@ extern "C" int cmaq_ros3_run_steps_device(cmaq_ros3_ctx *c, int64_t n, double ch
     float ms = 0.f;
     HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
     c->steps_ms = ms;
-    return check_fail(c, small, fail);
+    const int rc = check_fail(c, small, fail);
+    note_fail_inputs(c, n, temp, pres, h2o, rj, sea, c->d_rki);
+    return rc;
 }
 
 extern "C" int cmaq_ros3_last_steps_timing(const cmaq_ros3_ctx *c, double *ms)
@@ -1251,6 +1361,7 @@ extern "C" int cmaq_ros3_chem_device(cmaq_ros3_ctx *c, const cmaq_ros3_grid *g, 
     int rc = cmaq_ros3_run_device(c, n, chemstep, b[0], b[1], b[2], b[4], b[5], b[3], b[6], nacc, nrej,
                                   fail, stream);
     if (rc != 0 && rc != CMAQ_ROS3_E_CONVERGENCE && rc != CMAQ_ROS3_E_STEPS) return rc;
+    if (c->fd_code != 0) c->fd_met[4] = fetch_cell(g->dens, n, 0, c->fd_cell);   // BLKDENS
     if (int rs = cmaq_ros3_scatter_device(c, g, b[6], stream)) return rs;
     HIP_TRY(c, hipStreamSynchronize((hipStream_t)stream));
     return rc;

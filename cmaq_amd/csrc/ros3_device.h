@@ -252,6 +252,7 @@ struct SolveParams {
     int64_t *fail_cell;
     double *fail_info;        // maxerr, dt, tnow
     int32_t *fail_code;
+    double *fail_detail;      // first failure: ERR, then Y, YP, K1, K2, K3 [ns] (solver order), or nullptr
     int32_t max_attempts;
     int32_t refill_min;       // idle lanes needed before a wave refills
     // adjoint: per-cell step tapes (records of Y, K1, K2, K3, h per accepted step)

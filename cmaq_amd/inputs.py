@@ -9,12 +9,10 @@ Distributions (per cell):
   H2O log-uniform on [1e3, 3e4] ppm;
   every species log-uniform on [1e-6, 1e2] ppb, except O3 ~ U(20, 80) ppb and
   CO ~ U(80, 300) ppb; concentrations are passed in ppm;
-  species whose chemical lifetime is shorter than the solver's minimum step
-  DTMIN = 1e-8 min (O1D in every mechanism, ~2e-11 min) start at 1e-6 ppb:
-  for them ROS3's embedded error estimate tends to y0/2 (it is not damped as
-  h*lambda -> -inf), so a larger start value makes RBSOLVER reject down to
-  DTMIN and stop (rbsolver.F:528-583) -- with the plain distribution the
-  reference fails in ~58 % of cells;
+  short-lived species (lifetime below 0.6 ms: O1D, O3P, CB6's ROR, SAPRC07's
+  TBUO, CRACMM1-AMORE's ISON; see FAST_SPECIES) start at 1e-6 ppb -- with the
+  plain distribution RBSOLVER rejects down to DTMIN and stops
+  (rbsolver.F:528-583) in ~58 % of cells (O1D) or ~0.1 % (ROR);
   J = clear-sky profile (CLEAR_SKY_J, 1/min) * cos(zenith), cos(zenith) ~ U(0,1),
   with 30 % of cells at night (J = 0);
   heterogeneous rates and the sea-water fraction are 0.
@@ -104,18 +102,38 @@ VOC_SPECIES = {
                 "SESQ", "STY", "NAPH", "FURAN", "BDE13", "ACRO", "PROG"],
 }
 
-# species with a chemical lifetime below DTMIN (1e-8 min) somewhere in the
-# sampled atmospheric states: O(1D), loss frequency ~6.6e10 /min in every
-# mechanism of the CMAQ tree, and in CRACMM1-AMORE also ISON (up to 3.5e8 /min;
-# in the other mechanisms that carry ISON it is slow).  The next fastest,
-# O(3P), is ~1e7 /min.  Such a species starts at 1e-6 ppb instead of the
-# log-uniform draw, as it would after any earlier step.
-SUB_DTMIN_SPECIES = {"CRACMM1AMORE_AQ": ("O1D", "ISON")}
-SUB_DTMIN_DEFAULT = ("O1D",)
+# Short-lived species start at 1e-6 ppb (the bottom of the log-uniform range)
+# instead of the draw, as they would after any earlier step:
+# * every species whose first-order loss frequency (unimolecular and fixed-
+#   species reactions, photolysis at the clear-sky profile) exceeds 1e5 /min,
+#   i.e. a lifetime below 0.6 ms, anywhere in T in [250, 310] K, P in
+#   [500, 1013] hPa: O(1D) (~6.6e10 /min, below DTMIN = 1e-8 min), O(3P)
+#   (~1e7 /min), the CB6 alkoxy radical ROR (~5e6 /min) and SAPRC07's TBUO
+#   (~1.7e5 /min);
+# * CRACMM1-AMORE's ISON, whose bimolecular loss reaches 3.5e8 /min.
+# With O(1D) drawn up to 100 ppb ROS3's embedded error estimate tends to y0/2
+# (it is not damped as h*lambda -> -inf) and RBSOLVER rejects down to DTMIN
+# and stops (rbsolver.F:528-583) in ~58 % of cells.  With ROR drawn up to
+# 100 ppb its decomposition (R133, k ~ 1.5e6 /min) removes 2.7 PAR per ROR
+# through CB6's negative PAR coefficient within microseconds; where that
+# exceeds the PAR present, PAR's error term takes > 95 % of ERR and the same
+# failure follows in ~0.1 % of cells.  The list is re-derived from the rate
+# constants by tests/test_inputs.py.
+FAST_SPECIES = {
+    "CB6R5_AE7_AQ": ("O1D", "O", "ROR"),
+    "CB6R3_AE7_AQ": ("O1D", "O", "ROR"),
+    "CB6R5M_AE7_AQ": ("O1D", "O", "ROR"),
+    "CRACMM1AMORE_AQ": ("O1D", "O3P", "ISON"),
+    "SAPRC07TC_AE6_AQ": ("O1D", "O3P", "TBUO"),
+    "SAPRC07TIC_AE7I_AQ": ("O1D", "O3P", "TBUO"),
+}
+FAST_DEFAULT = ("O1D", "O3P")
+FAST_LOSS_PER_MIN = 1.0e5
 
 
 def sub_dtmin_species(mech):
-    return set(SUB_DTMIN_SPECIES.get(mech["mechname"], SUB_DTMIN_DEFAULT))
+    """Species started at 1e-6 ppb (the short-lived set above)."""
+    return set(FAST_SPECIES.get(mech["mechname"], FAST_DEFAULT))
 
 F_T, F_P, F_H2O, F_COSZ, F_NIGHT, F_REGIME, F_NO, F_NO2, F_SPC = 1, 2, 3, 4, 5, 6, 7, 8, 100
 

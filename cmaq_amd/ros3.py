@@ -59,6 +59,13 @@ class _Fail(C.Structure):
                 ("dt", C.c_double), ("tnow", C.c_double)]
 
 
+class _FailDetail(C.Structure):
+    _fields_ = [("code", C.c_int32), ("numb_mech_spc", C.c_int32), ("nrxns", C.c_int32),
+                ("nphotab", C.c_int32), ("cell", C.c_int64), ("maxerr", C.c_double), ("err", C.c_double),
+                ("dt", C.c_double), ("tnow", C.c_double), ("temp_k", C.c_double), ("pres_atm", C.c_double),
+                ("h2o_ppm", C.c_double), ("seawater", C.c_double), ("dens", C.c_double)]
+
+
 class _Grid(C.Structure):
     _fields_ = [("ncols", C.c_int32), ("nrows", C.c_int32), ("nlays", C.c_int32), ("ld_het", C.c_int32),
                 ("cgrid", C.c_void_p), ("ta", C.c_void_p), ("dens", C.c_void_p), ("pres", C.c_void_p),
@@ -101,6 +108,7 @@ def lib():
         L.cmaq_ros3_set_queue_order.argtypes = [vp, vp, C.c_int64]
         L.cmaq_ros3_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
         L.cmaq_ros3_species_order.argtypes = [vp, _i32p]
+        L.cmaq_ros3_get_fail_detail.argtypes = [vp, C.POINTER(_FailDetail)] + [_f64p] * 5
         L.cmaq_ros3_last_error.argtypes = [vp]
         L.cmaq_ros3_last_error.restype = C.c_char_p
         L.cmaq_ros3_free.argtypes = [vp]
@@ -238,6 +246,25 @@ class Ros3Solver:
         a, b = C.c_double(), C.c_double()
         lib().cmaq_ros3_last_timing(self._ctx, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def fail_detail(self):
+        """The first failure of the last call as rbsolver.F:532-583 reports it:
+        None without a failure, else a dict with the scalars of
+        cmaq_ros3_fail_detail and, in mechanism order, 'spc_err' (MAX_SPC_ERR),
+        'y' (last accepted state), 'yp' (the rejected step's state), plus the
+        cell's 'rj' and 'rki' (NaN where the call had no such input)."""
+        d = _FailDetail()
+        a = {"spc_err": np.zeros(self.ns), "y": np.zeros(self.ns), "yp": np.zeros(self.ns),
+             "rj": np.zeros(self.nphotab), "rki": np.zeros(self.nr)}
+        p = lambda k: a[k].ctypes.data_as(_f64p)
+        rc = lib().cmaq_ros3_get_fail_detail(self._ctx, C.byref(d), p("spc_err"), p("y"), p("yp"), p("rj"), p("rki"))
+        if rc != 0:
+            raise self._err(rc)
+        if d.code == 0:
+            return None
+        out = {k: getattr(d, k) for k, _ in _FailDetail._fields_}
+        out.update(a)
+        return out
 
     def species_order(self):
         out = np.zeros(self.ns, np.int32)

@@ -92,6 +92,26 @@ typedef struct cmaq_ros3_fail {
     double tnow;                /* elapsed minutes in the sync step                */
 } cmaq_ros3_fail;
 
+/*
+ * The first failure of the last call in full: what RBSOLVER writes to LOGDEV
+ * before M3EXIT (rbsolver.F:532-583).  temp_k .. seawater are the failing
+ * cell's BLKTEMP, BLKPRES, BLKCH2O, BLKSEAWATER and dens its DENS (kg/m3,
+ * BLKDENS); each is NaN where the call did not take that input (dens is set
+ * by the CHEM-level calls only; the met fields are not known to
+ * cmaq_ros3_solve_device and the adjoint calls).
+ */
+typedef struct cmaq_ros3_fail_detail {
+    int32_t code;               /* as cmaq_ros3_fail; 0 = the last call had no failure */
+    int32_t numb_mech_spc;      /* lengths of the arrays cmaq_ros3_fail_detail fills */
+    int32_t nrxns;
+    int32_t nphotab;
+    int64_t cell;               /* failing cell (0-based), -1 if none              */
+    double maxerr;              /* MAXERR of the last step                         */
+    double err;                 /* ERR(cell): sum of the squared scaled errors     */
+    double dt, tnow;            /* minutes                                         */
+    double temp_k, pres_atm, h2o_ppm, seawater, dens;
+} cmaq_ros3_fail_detail;
+
 typedef struct cmaq_ros3_ctx cmaq_ros3_ctx;
 
 /*
@@ -366,6 +386,19 @@ int cmaq_ros3_query(const cmaq_ros3_ctx *ctx, const char *key, int64_t *value);
 
 /* Solver species order: inew2old[new] = old (both 0-based), numb_mech_spc entries. */
 int cmaq_ros3_species_order(const cmaq_ros3_ctx *ctx, int32_t *inew2old);
+
+/*
+ * Details of the first failing cell of the last run/solve/CHEM call on this
+ * context (rbsolver.F:532-583 writes the same for CELL_MAXERR).  Per species,
+ * in mechanism (CHEMISTRY_SPC) order: spc_err = MAX_SPC_ERR, the species'
+ * share of ERR, ((D1*K1 + D2*K2 + D3*K3) / (ATOL + RTOL*|YP|))**2 / ERR
+ * (rbsolver.F:551-555); y = the last accepted state Y; yp = the rejected
+ * step's new state YP.  rj [nphotab] = RJBLK and rki [nrxns] = RKI of the
+ * cell (NaN where the call had no such input).  Any array may be NULL.  With
+ * no failure d->code = 0 and the arrays are NaN.
+ */
+int cmaq_ros3_get_fail_detail(const cmaq_ros3_ctx *ctx, cmaq_ros3_fail_detail *d, double *spc_err,
+                              double *y, double *yp, double *rj, double *rki);
 
 /* Message for the last failure on this context (static storage, never NULL). */
 const char *cmaq_ros3_last_error(const cmaq_ros3_ctx *ctx);

@@ -37,7 +37,8 @@ class _Tables(C.Structure):
 
 
 class _Fail(C.Structure):
-    _fields_ = [("cell", C.c_int64), ("maxerr", C.c_double), ("dt", C.c_double), ("tnow", C.c_double)]
+    _fields_ = [("cell", C.c_int64), ("maxerr", C.c_double), ("dt", C.c_double), ("tnow", C.c_double),
+                ("dump", C.c_void_p)]
 
 
 def build():
@@ -214,15 +215,23 @@ class Oracle:
         lib().om_solve(self._ctx, ncsp, rhs.shape[1], _p(cc2, _f64p), _p(vdiag, _f64p), _p(rhs, _f64p))
         return rhs
 
-    def rbsolver(self, ncsp, chemstep, rki, y_new, rtol_new, atol_new):
+    def rbsolver(self, ncsp, chemstep, rki, y_new, rtol_new, atol_new, dump=False):
+        """RBSOLVER on one block (arrays in NEW order).  With dump=True a
+        fifth value holds, after a DT < DTMIN failure, the failing cell's
+        ERR, Y, YP, K1, K2, K3 (rbsolver.F:540-560), else None."""
         rki = np.ascontiguousarray(rki, np.float64)
         y = np.array(y_new, np.float64, copy=True, order="C")
         rt = np.ascontiguousarray(rtol_new, np.float64)
         at = np.ascontiguousarray(atol_new, np.float64)
         na, nj, f = C.c_int32(), C.c_int32(), _Fail()
+        buf = np.full(1 + 5 * self.ns, np.nan)
+        if dump:
+            f.dump = buf.ctypes.data
         rc = lib().om_rbsolver(self._ctx, ncsp, y.shape[1], chemstep, _p(rki, _f64p), _p(y, _f64p),
                                _p(rt, _f64p), _p(at, _f64p), C.byref(na), C.byref(nj), C.byref(f))
-        return rc, y, na.value, nj.value
+        if not dump:
+            return rc, y, na.value, nj.value
+        return rc, y, na.value, nj.value, (buf if rc != 0 else None)
 
     def run(self, chemstep, temp, pres_atm, h2o_ppm, rj, y, rtol, atol, het=None, seawater=None, blksize=1):
         """Whole-domain driver; y is [ns, ncells] in mechanism order (copied)."""

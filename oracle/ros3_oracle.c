@@ -687,6 +687,25 @@ int om_rbsolver(const om_ctx *c, int ncsp, int nc, double chemstep, const double
             lpass = 0;
             if (dt < DTMIN) {
                 if (fail) { fail->maxerr = maxerr; fail->dt = dt; fail->tnow = tnow; }
+                if (fail && fail->dump) {
+                    /* CELL_MAXERR: the first cell holding the block maximum
+                       (rbsolver.F:446-455), then its ERR, Y, YP, K1..K3 */
+                    int cm = 0;
+                    double best = UROUND;
+                    for (int cc = 0; cc < nc; ++cc) {
+                        double s = sqrt(err[cc] * rnspec);
+                        if (s > best) { best = s; cm = cc; }
+                    }
+                    double *d = fail->dump;
+                    d[0] = err[cm];
+                    for (int n = 0; n < ns; ++n) {
+                        d[1 + n] = A2(y, n, cm);
+                        d[1 + ns + n] = A2(yp, n, cm);
+                        d[1 + 2 * ns + n] = A2(k1, n, cm);
+                        d[1 + 3 * ns + n] = A2(k2, n, cm);
+                        d[1 + 4 * ns + n] = A2(k3, n, cm);
+                    }
+                }
                 rc = -1;
                 break;
             }
@@ -706,6 +725,7 @@ int om_run(const om_ctx *c, int64_t ncells, int blksize, double chemstep,
     const int ns = c->ns, nr = c->nr;
     if (fail) { fail->cell = -1; fail->maxerr = 0; fail->dt = 0; fail->tnow = 0; }
     if (ncells <= 0) return 0;
+    if (fail && fail->dump && blksize != 1) return -3;   /* the dump names one cell */
     if (blksize < 1) return -3;
     /* block partition (rbinit.F:286-303) */
     int64_t nblks = 1 + (ncells - 1) / blksize;
@@ -738,6 +758,7 @@ int om_run(const om_ctx *c, int64_t ncells, int blksize, double chemstep,
         int ncsp = sunlit ? 1 : 2;
         int32_t na = 0, nj = 0;
         om_fail f1;
+        f1.dump = (rc == 0 && fail) ? fail->dump : NULL;
         int brc = om_rbsolver(c, ncsp, nc, chemstep, brk, by, rtn, atn, &na, &nj, &f1);
         if (brc != 0) {
             if (rc == 0 && fail) { *fail = f1; fail->cell = off; }

@@ -45,6 +45,8 @@ typedef struct om_fail {
     double maxerr;    /* MAXERR when DT fell below DTMIN                       */
     double dt;        /* DT after the last reduction                          */
     double tnow;      /* elapsed minutes when the failure happened            */
+    double *dump;     /* NULL, or [1 + 5 ns]: ERR, Y, YP, K1, K2, K3 (new order) of
+                         the failing cell, the values rbsolver.F:540-560 writes  */
 } om_fail;
 
 /* RBSPARSE (rbsparse.F:26-696).  Returns NULL and fills err on failure. */

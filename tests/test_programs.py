@@ -22,7 +22,8 @@ def _cell(mech, o, seed):
 
 @pytest.mark.parametrize("jac_order,fuse", [("column", False), ("row", False), ("group3", False),
                                             ("tile10x10", False), ("tile6x10", False), ("column", True),
-                                            ("tile10x10", True), ("row", True), ("group3", True)])
+                                            ("tile10x10", True), ("row", True), ("group3", True),
+                                            ("ctile10x10", True), ("ctile14x10", True), ("ctile12x12", False)])
 def test_forward_programs_bit_exact(cb6r5, oracle_cb6r5, jac_order, fuse):
     """Every LU schedule keeps each entry's reference operation sequence, so
     all reproduce the oracle bit for bit; so does stage 1's forward
