@@ -434,6 +434,7 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w(f"    (const void *)&ros3::{cname}::solve_kernel_t<false>, ros3::{cname}::INEW2OLD_HOST,")
     w(f"    \"{variant}\", {budget}, {lookahead}, ros3::{cname}::WAVES_PER_EU, {1 if is_default else 0}, {refill_min},")
     w(f"    (const void *)&ros3::{cname}::solve_kernel_t<true>, ROS3_REVERSE_KERNEL,")
-    w(f"    ros3::{cname}::TAPE_RS, (const void *)&ros3::{cname}::solve_kernel_t<false, true>}};")
+    w(f"    ros3::{cname}::TAPE_RS, (const void *)&ros3::{cname}::solve_kernel_t<false, true>,")
+    w(f"    ros3::{cname}::OFF_LAM}};   // rows of the workspace the forward kernel touches")
     w("#undef ROS3_REVERSE_KERNEL")
     return ("\n".join(out) + "\n", "\n".join(adj) + "\n"), sym

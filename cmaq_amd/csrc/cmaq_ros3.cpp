@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,22 @@ constexpr int kMaxMul = 64;
 constexpr int kSolveBlock = 256;
 
 thread_local std::string g_last_error = "";
+
+// Scratch of the allocation-free entry point (cmaq_ros3_solve_device_noalloc):
+// a fixed device array per GPU, holding at kScratchSmall the cell queue and
+// fail record (the layout of the context's small buffer), at
+// kScratchDetail the failing cell's dump and from kScratchWs the solver
+// workspace of as many waves as fit.  In host code `scratch` names a
+// host-side stand-in, never device memory: the per-device address comes from
+// hipGetSymbolAddress (g_scratch) and is the only one passed to kernels.
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+constexpr size_t kScratchSmall = 0;
+constexpr size_t kScratchDetail = 256;
+constexpr size_t kScratchWs = (size_t)1 << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
 
 }  // namespace
 
@@ -69,6 +86,7 @@ struct cmaq_ros3_ctx {
     std::vector<double> fd_raw;                    // 1 + 5 * ns doubles, solver order
     double fd_met[5] = {NAN, NAN, NAN, NAN, NAN};  // temp, pres, h2o, seawater, dens
     std::vector<double> fd_rj, fd_rki;
+    unsigned char *scratch_dev = nullptr;          // this device's scratch array (device address)
     double *d_rki = nullptr;
     int64_t rki_cells = 0;
     // adjoint tapes
@@ -465,6 +483,19 @@ extern "C" int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech, c
         c->d_fail_flag = (int32_t *)((char *)c->d_small + 40);
         c->d_fail_code = (int32_t *)((char *)c->d_small + 44);
         for (hipEvent_t &e : c->ev) INIT_TRY(hipEventCreate(&e));
+        {   // the scratch array's address on this device, once per device
+            std::lock_guard<std::mutex> lock(g_scratch_mu);
+            if (dev >= kMaxDevices) { rc = set_err(c, CMAQ_ROS3_E_HIP, "device ordinal above 63"); goto fail; }
+            if (!g_scratch[dev]) {
+                void *ptr = nullptr;
+                INIT_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+                g_scratch[dev] = ptr;
+            }
+            c->scratch_dev = (unsigned char *)g_scratch[dev];
+        }
+        c->fd_raw.reserve(1 + 5 * (size_t)c->ns);
+        c->fd_rj.reserve(c->nphotab);
+        c->fd_rki.reserve(c->nr);
         INIT_TRY(hipMemcpy(c->d_ops, c->ops.data(), c->ops.size() * sizeof(RateOp), hipMemcpyHostToDevice));
         if (!c->mul.empty())
             INIT_TRY(hipMemcpy(c->d_mul, c->mul.data(), c->mul.size() * sizeof(RateMul), hipMemcpyHostToDevice));
@@ -657,7 +688,10 @@ static const char *fail_text(int code)
 }
 
 // Reads the fail record after the stream has finished; returns its code.
-static int check_fail(cmaq_ros3_ctx *c, const unsigned char *small, cmaq_ros3_fail *fail)
+// detail: the device dump of the failing cell (the context's, or the
+// allocation-free entry point's in its scratch array).
+static int check_fail(cmaq_ros3_ctx *c, const unsigned char *small, cmaq_ros3_fail *fail,
+                      const double *detail = nullptr)
 {
     int32_t flag, code;
     int64_t cell;
@@ -673,7 +707,7 @@ static int check_fail(cmaq_ros3_ctx *c, const unsigned char *small, cmaq_ros3_fa
         c->fd_cell = cell;
         std::memcpy(c->fd_info, info, sizeof info);
         c->fd_raw.assign(1 + 5 * (size_t)c->ns, NAN);
-        if (hipMemcpy(c->fd_raw.data(), c->d_fail_detail, c->fd_raw.size() * sizeof(double),
+        if (hipMemcpy(c->fd_raw.data(), detail ? detail : c->d_fail_detail, c->fd_raw.size() * sizeof(double),
                       hipMemcpyDeviceToHost) != hipSuccess)
             c->fd_raw.assign(c->fd_raw.size(), NAN);
     }
@@ -1217,6 +1251,60 @@ extern "C" int cmaq_ros3_run_steps_device(cmaq_ros3_ctx *c, int64_t n, double ch
     c->steps_ms = ms;
     const int rc = check_fail(c, small, fail);
     note_fail_inputs(c, n, temp, pres, h2o, rj, sea, c->d_rki);
+    return rc;
+}
+
+extern "C" int cmaq_ros3_solve_device_noalloc(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,
+                                              double *y, int32_t *nacc, int32_t *nrej, cmaq_ros3_fail *fail,
+                                              void *stream)
+{
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (n > 0 && (!rki || !y)) return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    if (!c->scratch_dev) return set_err(c, CMAQ_ROS3_E_HIP, "no scratch array on this device");
+    if (kScratchDetail + (1 + 5 * (size_t)c->ns) * sizeof(double) > kScratchWs)
+        return set_err(c, CMAQ_ROS3_E_NOMEM, "too many species for the scratch array's failure record");
+    // waves whose workspace (the forward kernel's rows only) fits, in whole workgroups
+    const int wpb = kSolveBlock / kWave;
+    const int64_t stride = (((int64_t)c->mk->nws_fwd + 7) / 8 * 8) * kWave;   // doubles per wave
+    const int64_t fit = (int64_t)((kScratchBytes - kScratchWs) / ((size_t)stride * sizeof(double)));
+    const int64_t blocks = std::min<int64_t>(c->grid_blocks, fit / wpb);
+    if (blocks < 1)
+        return set_err(c, CMAQ_ROS3_E_NOMEM, "one workgroup's solver workspace (" +
+                                                 std::to_string(stride * wpb * 8) + " B) exceeds the " +
+                                                 std::to_string(kScratchBytes - kScratchWs) +
+                                                 " B scratch array");
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    unsigned char *base = c->scratch_dev;
+    SolveParams P;
+    solve_params(c, P, n, chemstep, rki, y, nacc, nrej);
+    if (c->user_order && c->user_order_n == n) P.order = c->user_order;
+    P.ws = (double *)(base + kScratchWs);
+    P.ws_stride = stride;
+    P.queue = (unsigned long long *)(base + kScratchSmall);
+    P.fail_cell = (int64_t *)(base + kScratchSmall + 8);
+    P.fail_info = (double *)(base + kScratchSmall + 16);
+    P.fail_flag = (int32_t *)(base + kScratchSmall + 40);
+    P.fail_code = (int32_t *)(base + kScratchSmall + 44);
+    P.fail_detail = (double *)(base + kScratchDetail);
+    // the queue and fail record start at zero: cleared by a kernel, since the
+    // array keeps the previous call's contents
+    hipLaunchKernelGGL(small_reset_kernel, dim3(1), dim3(64), 0, s, (unsigned long long *)(base + kScratchSmall));
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipEventRecord(c->ev[1], s));
+    void *args[] = {&P};
+    HIP_TRY(c, hipLaunchKernel(c->mk->solve_kernel, dim3((unsigned)blocks), dim3(kSolveBlock), args, 0, s));
+    HIP_TRY(c, hipEventRecord(c->ev[2], s));
+    unsigned char small[64];
+    HIP_TRY(c, hipMemcpyAsync(small, base + kScratchSmall, 64, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    float ms_s = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
+    c->rates_ms = 0.0;
+    c->solve_ms = ms_s;
+    const int rc = check_fail(c, small, fail, P.fail_detail);
+    note_fail_inputs(c, n, nullptr, nullptr, nullptr, nullptr, nullptr, rki);
     return rc;
 }
 

@@ -20,6 +20,7 @@ struct MechKernels {
     const void *reverse_kernel;       // discrete adjoint over the tapes
     int tape_rs;                      // doubles per tape record
     const void *tlm_kernel;           // solve_kernel carrying a tangent-linear direction
+    int nws_fwd;                      // workspace rows (entries) the forward kernel touches
 };
 
 }  // namespace ros3

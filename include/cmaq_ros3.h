@@ -191,6 +191,26 @@ int cmaq_ros3_solve_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_m
                            int32_t *n_rejected, cmaq_ros3_fail *fail, void *stream);
 
 /*
+ * Allocation-free variant of cmaq_ros3_solve_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * stream capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results, bit for bit; the call makes no hipMalloc, hipFree
+ * or other allocating call.  Its solver workspace, cell queue and failure
+ * record live in one static device array of 256 MiB per GPU, held until the
+ * process ends, instead of the context's buffers: the persistent kernel runs
+ * with as many waves as that array holds (the forward's workspace rows x 512
+ * B per wave in 1 MiB less than 256 MiB: 152 waves for CB6r5), so it is
+ * slower than cmaq_ros3_solve_device.  A mechanism whose one workgroup (4
+ * waves) does not fit is refused with CMAQ_ROS3_E_NOMEM.  The array keeps the
+ * last call's contents and is shared by every context on the GPU: calls that
+ * overlap on one GPU race.  The context (cmaq_ros3_init, which allocates) is
+ * created beforehand.
+ */
+int cmaq_ros3_solve_device_noalloc(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep,
+                                   const double *rki, double *y_ppm, int32_t *n_accepted,
+                                   int32_t *n_rejected, cmaq_ros3_fail *fail, void *stream);
+
+/*
  * Discrete adjoint of one cmaq_ros3_solve_device call (no counterpart in the
  * reference, whose sensitivity option is DDM-3D, ddm3d/DDM3D_CHEM.F:155-511).
  * Given the call's inputs -- rki, the initial state y0 [ns][ncells] and the
