@@ -20,8 +20,9 @@ stores every accepted step (Y, K1, K2, K3, h) in one arena of up to
 and the backward runs only the reverse sweeps (cmaq_ros3_reverse_device).  If
 the arena fills up, or with ``tape="replay"``, the stage vectors are instead
 recomputed within each sync step by cmaq_ros3_adjoint_device (replay into
-per-step tapes, then the reverse sweep).  The arena is released, and a large
-one returned to the device, when the backward pass has run.
+per-step tapes, then the reverse sweep).  The solver keeps the arena after the
+backward pass and reuses it for its next integration (sized from this one's
+record count); ``release_tape(solver)`` returns it to the device.
 """
 
 import torch
@@ -41,14 +42,7 @@ class Ros3Integrate(torch.autograd.Function):
         rk = rki.detach().contiguous()
         arena = None
         if tape == "stored" and (ctx.needs_input_grad[0] or ctx.needs_input_grad[1]) and n > 0:
-            free, _ = torch.cuda.mem_get_info(y.device)
-            rs = solver.query("tape_record_doubles")
-            cap = min(int(free * tape_fraction) // (8 * rs), n * nsteps * 64)
-            if tape_records is not None:
-                cap = min(cap, int(tape_records))
-            # skip the attempt when even the fewest possible steps cannot fit
-            if n * nsteps * min_accepted_steps(chemstep) <= cap:
-                arena = solver.new_tape(cap, device=y.device)
+            arena = _take_arena(solver, y.device, n, nsteps, chemstep, tape_fraction, tape_records)
         states, counts, heads = [], [], []
         for _ in range(nsteps):
             states.append(y.clone())
@@ -60,11 +54,13 @@ class Ros3Integrate(torch.autograd.Function):
                 if solver.solve_tape_device(chemstep, rk, y, na, *arena, head, raise_on_fail=False):
                     heads.append(head)
                 else:       # arena full: the step is complete, the backward replays
+                    _give_back(solver, arena, overflowed=True)
                     arena, heads = None, []
-                    _release(True)
             else:
                 solver.solve_device(chemstep, rk, y, na, raise_on_fail=False)
             counts.append(na)
+        if arena is not None:     # sizes the next integration's arena
+            solver._tape_used = (int(arena[1].item()), n, nsteps)
         ctx.arena, ctx.heads = arena, heads
         ctx.solver, ctx.chemstep = solver, chemstep
         ctx.save_for_backward(rk, *states, *counts)
@@ -97,10 +93,8 @@ class Ros3Integrate(torch.autograd.Function):
             for s in range(k - 1, -1, -1):
                 ctx.solver.adjoint_device(ctx.chemstep, rk, states[s], counts[s], lam, grad_rki)
         if arena is not None:
-            big = arena[0].numel() * 8 > (8 << 30)
             ctx.arena, ctx.heads = None, []
-            del arena, heads
-            _release(big)
+            _give_back(ctx.solver, arena)
         return (lam if ctx.needs_input_grad[0] else None), grad_rki, None, None, None, None, None, None
 
 
@@ -120,10 +114,66 @@ def min_accepted_steps(chemstep, dtstart=0.05, dtmax=10.0, facmax=10.0):
     return k
 
 
-def _release(big):
-    """Return a large tape arena's memory to the device once it is freed."""
-    if big:
+def _take_arena(solver, device, n, nsteps, chemstep, tape_fraction, tape_records):
+    """A stored-tape arena (tape, count) with its count at zero, or None.
+
+    The solver keeps one arena between integrations and reuses it, so that an
+    optimisation loop neither allocates nor returns ~100 GB per call.  Its
+    size comes from the previous integration's record count (plus 25 % and one
+    record per cell and sync step) when that had the same cells and sync
+    steps, else from the old upper bound (tape_fraction of the free memory, at
+    most 64 records per cell and sync step).  An arena still held by an
+    integration whose backward has not run is not shared: the call then gets
+    a fresh one."""
+    rs = solver.query("tape_record_doubles")
+    used = getattr(solver, "_tape_used", None)
+    want = None
+    if used is not None and used[1:] == (n, nsteps):
+        want = int(used[0] * 1.25) + n * nsteps
+    held = getattr(solver, "_arena", None)
+    if held is not None and not getattr(solver, "_arena_busy", False):
+        cap = held[0].numel() // rs
+        if (want is None or want <= cap <= 2 * want) and (tape_records is None or cap <= int(tape_records)) \
+                and cap >= n * nsteps * min_accepted_steps(chemstep):
+            held[1].zero_()
+            solver._arena_busy = True
+            return held
+        solver._arena = None            # too small or too large: replace it
+        del held
         torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info(device)
+    cap = min(int(free * tape_fraction) // (8 * rs), n * nsteps * 64)
+    if want is not None:
+        cap = min(cap, want)
+    if tape_records is not None:
+        cap = min(cap, int(tape_records))
+    # skip the attempt when even the fewest possible steps cannot fit
+    if n * nsteps * min_accepted_steps(chemstep) > cap:
+        return None
+    arena = solver.new_tape(cap, device=device)
+    if getattr(solver, "_arena_busy", False):
+        return arena                    # private to this integration
+    solver._arena, solver._arena_busy = arena, True
+    return arena
+
+
+def _give_back(solver, arena, overflowed=False):
+    """End of an integration's use of its arena: the solver's own arena is
+    kept for the next call (an overflowed one is dropped); a private one is
+    freed."""
+    if overflowed:
+        solver._tape_used = None        # the next arena comes from the upper bound
+    if getattr(solver, "_arena", None) is arena:
+        solver._arena_busy = False
+        if overflowed:
+            solver._arena = None
+            torch.cuda.empty_cache()
+
+
+def release_tape(solver):
+    """Return the solver's kept stored-tape arena to the device."""
+    solver._arena, solver._arena_busy = None, False
+    torch.cuda.empty_cache()
 
 
 def integrate(solver, y0, rki, chemstep=5.0, nsteps=1, tape="stored", tape_fraction=0.7, tape_records=None):

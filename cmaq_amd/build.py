@@ -68,7 +68,14 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contr
 # the long straight-line functions (SAPRC07TIC: 637 s -> 227 s) and the
 # kernels run as fast without them (CB6r5: 36.6 vs 36.5 M cell-steps/s).
 KERNEL_FLAGS = ["-mllvm", "-amdgpu-disable-unclustered-high-rp-reschedule",
-                "-mllvm", "-amdgpu-disable-clustered-low-occupancy-reschedule"]
+                "-mllvm", "-amdgpu-disable-clustered-low-occupancy-reschedule",
+                # no `tail` call markers: LLVM then lets every generated
+                # (internal, non-recursive) function clobber the callee-saved
+                # registers (interprocedural register allocation), instead of
+                # saving and restoring up to ~300 of them through scratch
+                # memory on every call -- which it does as soon as one call to
+                # it is marked `tail`
+                "-fno-optimize-sibling-calls"]
 
 
 def load_mech(name):

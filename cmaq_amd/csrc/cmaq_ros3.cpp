@@ -8,6 +8,7 @@
 //                   (rbdriver.F:440-677): one rates kernel, then the
 //                   persistent queue-refilled solver kernel.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -94,6 +95,8 @@ struct cmaq_ros3_ctx {
     size_t tape_bytes = 0;
     int64_t *d_tape_off = nullptr;
     int32_t *d_order = nullptr;       // adjoint queue order, longest cells first
+    void *d_sort = nullptr;           // keys, permutation and scratch of the device sort
+    size_t sort_bytes = 0;
     const int32_t *user_order = nullptr;   // caller's queue order for the forward (device)
     int64_t user_order_n = 0;
     int64_t tape_off_cells = 0;
@@ -342,6 +345,19 @@ int validate(const cmaq_ros3_mech *m, std::string &why)
     return 0;
 }
 
+// grow-only device buffer
+int grow_buffer(cmaq_ros3_ctx *c, void **buf, size_t *have, size_t need, const char *what)
+{
+    if (*have >= need) return 0;
+    hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+    if (hipMalloc(buf, need) != hipSuccess)
+        return set_err(c, CMAQ_ROS3_E_NOMEM, std::string("hipMalloc of ") + what + " failed");
+    *have = need;
+    return 0;
+}
+
 template <class T>
 int dalloc(cmaq_ros3_ctx *c, T **p, size_t n)
 {
@@ -518,7 +534,7 @@ extern "C" void cmaq_ros3_free(cmaq_ros3_ctx *c)
     hipSetDevice(c->device);
     hipFree(c->d_ops); hipFree(c->d_mul); hipFree(c->d_rtol); hipFree(c->d_atol);
     hipFree(c->d_ws); hipFree(c->d_small); hipFree(c->d_fail_detail); hipFree(c->d_rki); hipFree(c->d_io);
-    hipFree(c->d_tape); hipFree(c->d_tape_off); hipFree(c->d_order);
+    hipFree(c->d_tape); hipFree(c->d_tape_off); hipFree(c->d_order); hipFree(c->d_sort);
     hipFree(c->d_step_na); hipFree(c->d_step_nr);
     hipFree(c->d_map); hipFree(c->d_block); hipFree(c->d_grid);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
@@ -953,14 +969,20 @@ extern "C" int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *c, int64_t n, double chem
     return check_fail(c, small, fail);
 }
 
-// Cells longest first into c->d_order (allocated on demand), from the
-// forward's accepted-step counts already copied to the host.
-static int upload_longest_first(cmaq_ros3_ctx *c, const std::vector<int32_t> &na, hipStream_t s)
+namespace {
+__global__ void iota_kernel(int32_t *a, int64_t n)
 {
-    const int64_t n = (int64_t)na.size();
-    std::vector<int32_t> order((size_t)n);
-    for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return na[a] > na[b]; });
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = (int32_t)i;
+}
+}  // namespace
+
+// Cells longest first into c->d_order (allocated on demand): a stable radix
+// sort on the device of the forward's accepted-step counts nacc (device
+// array), descending, so equal counts keep ascending cell order.
+static int sort_longest_first(cmaq_ros3_ctx *c, const int32_t *nacc, int64_t n, hipStream_t s)
+{
+    if (n > INT32_MAX) return set_err(c, CMAQ_ROS3_E_ARG, "too many cells for the queue order");
     if (c->tape_off_cells < n) {
         hipFree(c->d_tape_off);
         hipFree(c->d_order);
@@ -972,8 +994,21 @@ static int upload_longest_first(cmaq_ros3_ctx *c, const std::vector<int32_t> &na
             return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of tape offsets failed");
         c->tape_off_cells = n;
     }
-    HIP_TRY(c, hipMemcpyAsync(c->d_order, order.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, s));
-    HIP_TRY(c, hipStreamSynchronize(s));
+    size_t temp = 0;
+    HIP_TRY(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp, nacc, (int32_t *)nullptr,
+                                                            (const int32_t *)nullptr, c->d_order, (int)n, 0,
+                                                            32, s));
+    // sorted keys and the input permutation, then the sort's own scratch
+    const size_t keys = ((size_t)n * sizeof(int32_t) + 255) & ~(size_t)255;
+    if (int rc = grow_buffer(c, &c->d_sort, &c->sort_bytes, 2 * keys + temp, "the queue-order sort")) return rc;
+    int32_t *keys_out = (int32_t *)c->d_sort;
+    int32_t *vals_in = (int32_t *)((char *)c->d_sort + keys);
+    void *tmp = (char *)c->d_sort + 2 * keys;
+    hipLaunchKernelGGL(iota_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0, s,
+                       vals_in, n);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipcub::DeviceRadixSort::SortPairsDescending(tmp, temp, nacc, keys_out, vals_in, c->d_order,
+                                                            (int)n, 0, 32, s));
     return 0;
 }
 
@@ -1028,14 +1063,12 @@ extern "C" int cmaq_ros3_reverse_device(cmaq_ros3_ctx *c, int64_t n, double chem
     if (n == 0) return 0;
     HIP_TRY(c, hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
-    std::vector<int32_t> na((size_t)n);
     uint64_t used = 0;
-    HIP_TRY(c, hipMemcpyAsync(na.data(), nacc_fwd, (size_t)n * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipMemcpyAsync(&used, tape_count, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
     if (used > (uint64_t)tape_cap)
         return set_err(c, CMAQ_ROS3_E_NOMEM, "the tape arena overflowed during the forward run");
-    if (int rc = upload_longest_first(c, na, s)) return rc;
+    if (int rc = sort_longest_first(c, nacc_fwd, n, s)) return rc;
     SolveParams P;
     solve_params(c, P, n, chemstep, rki, nullptr, nullptr, nullptr);
     P.tape = const_cast<double *>(tape);
@@ -1320,19 +1353,6 @@ extern "C" int cmaq_ros3_last_steps_timing(const cmaq_ros3_ctx *c, double *ms)
 // ---------------------------------------------------------------------------
 namespace {
 
-// grow-only device buffer
-int ensure_buffer(cmaq_ros3_ctx *c, void **buf, size_t *have, size_t need, const char *what)
-{
-    if (*have >= need) return 0;
-    hipFree(*buf);
-    *buf = nullptr;
-    *have = 0;
-    if (hipMalloc(buf, need) != hipSuccess)
-        return set_err(c, CMAQ_ROS3_E_NOMEM, std::string("hipMalloc of ") + what + " failed");
-    *have = need;
-    return 0;
-}
-
 int check_grid(cmaq_ros3_ctx *c, const cmaq_ros3_grid *g)
 {
     if (!c->d_map) return set_err(c, CMAQ_ROS3_E_ARG, "cmaq_ros3_set_cgrid_map has not been called");
@@ -1441,7 +1461,7 @@ extern "C" int cmaq_ros3_chem_device(cmaq_ros3_ctx *c, const cmaq_ros3_grid *g, 
     const size_t sizes[7] = {nd, nd, nd, nd, nd * c->nphotab, g->khetero ? nd * c->nhetero : 0, nd * c->ns};
     size_t need = 0;
     for (size_t b : sizes) need += al(b);
-    if (int rc = ensure_buffer(c, &c->d_block, &c->block_bytes, need, "CHEM block arrays")) return rc;
+    if (int rc = grow_buffer(c, &c->d_block, &c->block_bytes, need, "CHEM block arrays")) return rc;
     double *b[7];
     char *p = (char *)c->d_block;
     for (int i = 0; i < 7; ++i) { b[i] = sizes[i] ? (double *)p : nullptr; p += al(sizes[i]); }
@@ -1473,7 +1493,7 @@ extern "C" int cmaq_ros3_chem(cmaq_ros3_ctx *c, const cmaq_ros3_grid *g, double 
                               (size_t)n * 4};
     size_t need = 0;
     for (size_t b : sizes) need += al(b);
-    if (int rc = ensure_buffer(c, &c->d_grid, &c->grid_bytes, need, "CHEM grid arrays")) return rc;
+    if (int rc = grow_buffer(c, &c->d_grid, &c->grid_bytes, need, "CHEM grid arrays")) return rc;
     char *b[11];
     char *p = (char *)c->d_grid;
     for (int i = 0; i < 11; ++i) { b[i] = sizes[i] ? p : nullptr; p += al(sizes[i]); }

@@ -96,6 +96,7 @@ def lib():
         L.cmaq_ros3_run_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 9 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_rates_device.argtypes = [vp, C.c_int64] + [vp] * 7 + [vp]
         L.cmaq_ros3_solve_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 4 + [C.POINTER(_Fail), vp]
+        L.cmaq_ros3_solve_device_noalloc.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 4 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_adjoint_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 5 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_last_adjoint_timing.argtypes = [vp, _f64p, _f64p, C.POINTER(C.c_int64)]
         L.cmaq_ros3_solve_tape_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 5 + [C.c_int64] + \
@@ -485,6 +486,23 @@ class Ros3Solver:
             raise self._err(rc, fail)
         return fail if rc != 0 else None
 
+    def solve_device_noalloc(self, chemstep, rki, y, nacc=None, nrej=None, stream=None, raise_on_fail=True):
+        """solve_device through the allocation-free entry point (its
+        workspace in the library's static device array; fewer waves, same
+        results); see cmaq_ros3_solve_device_noalloc."""
+        import torch
+        n = y.numel() // self.ns
+        self._check([("rki", rki, self.nr), ("y", y, self.ns), ("nacc", nacc, 1), ("nrej", nrej, 1)], n)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_ros3_solve_device_noalloc(self._ctx, n, chemstep, _ptr(rki), _ptr(y), _ptr(nacc),
+                                                  _ptr(nrej), C.byref(f), C.c_void_p(stream))
+        fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, fail)
+        return fail if rc != 0 else None
+
     TLM_MAX_DIRECTIONS = 8
 
     def tlm_device(self, chemstep, rki, y, dy, dk=None, nacc=None, nrej=None, stream=None, raise_on_fail=True):
@@ -548,6 +566,8 @@ class Ros3Solver:
                                                _ptr(tape), cap, _ptr(tape_count), _ptr(tape_head), C.byref(f),
                                                C.c_void_p(stream))
         if rc == -6:          # the call allocates nothing: -6 is the arena overflow
+            if f.code != 0 and raise_on_fail:    # ... and a cell also failed
+                raise self._err(f.code, {"code": f.code, "cell": f.cell, "tape_overflow": True})
             return False
         if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
             raise self._err(rc, {"code": rc, "cell": f.cell})

@@ -76,6 +76,7 @@ def lib():
                                   _i32p, _i32p, C.POINTER(_Fail)]
         L.om_run.argtypes = [vp, C.c_int64, C.c_int, C.c_double] + [_f64p] * 7 + [_f64p, _f64p,
                                                                                _i32p, _i32p, C.POINTER(_Fail)]
+        L.om_stiffness.argtypes = [vp, C.c_int64, C.c_int] + [_f64p] * 10
         L.om_steps_cell.argtypes = [vp, C.c_int, C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_int,
                                     C.POINTER(C.c_int)]
         L.om_replay_cell.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, C.c_int]
@@ -251,6 +252,22 @@ class Oracle:
                           C.byref(f))
         return {"rc": rc, "y": yy, "nacc": na, "nrej": nj,
                 "fail": {"cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}}
+
+    def stiffness(self, temp, pres_atm, h2o_ppm, rj, y, rtol, atol, het=None, seawater=None, blksize=50):
+        """ERRMX2 of the LORDERING pre-pass for every cell (y [ns, ncells],
+        mechanism order)."""
+        temp = np.ascontiguousarray(temp, np.float64)
+        nc = temp.size
+        args = [np.ascontiguousarray(a, np.float64) for a in (pres_atm, h2o_ppm, rj, y)]
+        het = np.zeros((max(self.nhetero, 1), nc)) if het is None else np.ascontiguousarray(het, np.float64)
+        sw = np.zeros(nc) if seawater is None else np.ascontiguousarray(seawater, np.float64)
+        rt = np.ascontiguousarray(np.broadcast_to(rtol, (self.ns,)), np.float64)
+        at = np.ascontiguousarray(np.broadcast_to(atol, (self.ns,)), np.float64)
+        out = np.zeros(nc)
+        lib().om_stiffness(self._ctx, nc, blksize, _p(temp, _f64p), _p(args[0], _f64p), _p(args[1], _f64p),
+                           _p(args[2], _f64p), _p(het, _f64p), _p(sw, _f64p), _p(args[3], _f64p),
+                           _p(rt, _f64p), _p(at, _f64p), _p(out, _f64p))
+        return out
 
     # ---- discrete adjoint with the accepted steps held fixed (one cell) ----
     # y / lam arguments here are in MECHANISM order; the day pattern is used.

@@ -776,6 +776,55 @@ int om_run(const om_ctx *c, int64_t ncells, int blksize, double chemstep,
     return rc;
 }
 
+/* The LORDERING pre-pass of the ros3 CHEM driver (rbdriver.F:430-434,
+ * 440-521; rbsolver.F:346-360): for every cell, in blocks of blksize
+ * consecutive cells with the block's LSUNLIGHT deciding the pattern, one
+ * CALC_RCONST and one RBFEVAL at the initial state and the stiffness
+ * estimate ERRMX2 = sum over the ISCHAN species of
+ * (YDOT / (Y + ATOL / MIN(RTOL, 0.003)))^2.  The driver then heap-sorts the
+ * cells by ERRMX2 (rbdriver.F:683-724) and integrates them in that order.
+ * y is in MECHANISM order [ns][ncells]; rtol/atol per species in mechanism
+ * order; errmx2 [ncells] out. */
+void om_stiffness(const om_ctx *c, int64_t ncells, int blksize, const double *temp, const double *pres,
+                  const double *h2o, const double *rj, const double *het, const double *seawater,
+                  const double *y, const double *rtol, const double *atol, double *errmx2)
+{
+    const om_tables *t = &c->t;
+    const int ns = c->ns, nr = c->nr;
+    if (ncells <= 0 || blksize < 1) return;
+    double *bt = malloc(blksize * sizeof(double)), *bp = malloc(blksize * sizeof(double));
+    double *bh = malloc(blksize * sizeof(double)), *bs = malloc(blksize * sizeof(double));
+    double *brj = malloc((size_t)(t->nphotab ? t->nphotab : 1) * blksize * sizeof(double));
+    double *bhet = malloc((size_t)(t->nhetero ? t->nhetero : 1) * blksize * sizeof(double));
+    double *brk = malloc((size_t)nr * blksize * sizeof(double));
+    double *by = malloc((size_t)ns * blksize * sizeof(double));
+    double *bd = malloc((size_t)ns * blksize * sizeof(double));
+    for (int64_t off = 0; off < ncells; off += blksize) {
+        const int nc = (int)(ncells - off < blksize ? ncells - off : blksize);
+        for (int cc = 0; cc < nc; ++cc) {
+            bt[cc] = temp[off + cc]; bp[cc] = pres[off + cc]; bh[cc] = h2o[off + cc];
+            bs[cc] = seawater ? seawater[off + cc] : 0.0;
+            for (int i = 0; i < t->nphotab; ++i) brj[(size_t)i * nc + cc] = rj[(size_t)i * ncells + off + cc];
+            for (int i = 0; i < t->nhetero; ++i) bhet[(size_t)i * nc + cc] = het ? het[(size_t)i * ncells + off + cc] : 0.0;
+            for (int iold = 1; iold <= ns; ++iold)
+                by[(size_t)(c->iold2new[iold] - 1) * nc + cc] = y[(size_t)(iold - 1) * ncells + off + cc];
+        }
+        const int ncsp = om_rates_block(c, nc, bt, bp, bh, brj, bhet, bs, brk) ? 1 : 2;
+        om_feval(c, ncsp, nc, brk, by, bd);
+        for (int cc = 0; cc < nc; ++cc) errmx2[off + cc] = 0.0;
+        for (int j = 1; j <= c->ischan; ++j) {
+            const int old = c->inew2old[j] - 1;
+            const double rt = rtol[old] < 0.003 ? rtol[old] : 0.003;
+            const double ylow = atol[old] / rt;
+            for (int cc = 0; cc < nc; ++cc) {
+                const double e = A2(bd, j - 1, cc) / (A2(by, j - 1, cc) + ylow);
+                errmx2[off + cc] = errmx2[off + cc] + e * e;
+            }
+        }
+    }
+    free(bt); free(bp); free(bh); free(bs); free(brj); free(bhet); free(brk); free(by); free(bd);
+}
+
 /* ------------------------------------------------------------------ */
 /* Per-cell step tape, fixed-step replay and discrete adjoint          */
 /* ------------------------------------------------------------------ */

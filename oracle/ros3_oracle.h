@@ -99,6 +99,13 @@ int om_run(const om_ctx *c, int64_t ncells, int blksize, double chemstep,
            double *y, const double *rtol, const double *atol,
            int32_t *nacc, int32_t *nrej, om_fail *fail);
 
+/* LORDERING pre-pass (rbdriver.F:430-434, rbsolver.F:346-360): the stiffness
+ * estimate ERRMX2 of every cell from one CALC_RCONST + RBFEVAL per block of
+ * blksize cells; y [ns][ncells] and rtol/atol in mechanism order. */
+void om_stiffness(const om_ctx *c, int64_t ncells, int blksize, const double *temp, const double *pres,
+                  const double *h2o, const double *rj, const double *het, const double *seawater,
+                  const double *y, const double *rtol, const double *atol, double *errmx2);
+
 /* Discrete adjoint of the per-cell integration with its accepted-step
  * sequence held fixed (no reference counterpart; see ros3_oracle.c).
  * One cell, y / lam in NEW species order, rki [nr].

@@ -315,3 +315,26 @@ def test_integration_vs_radau(cb6r5, oracle_cb6r5):
         big = yref > 1e-6
         rel = np.abs(yr[big, 0] - yref[big]) / yref[big]
         assert rel.max() < 5e-3, rel.max()
+
+
+def test_stiffness_prepass_matches_feval(cb6r5, oracle_cb6r5):
+    """The LORDERING pre-pass (rbsolver.F:346-360): ERRMX2 of each cell is the
+    sum over species of (ydot / (y + ATOL / MIN(RTOL, 0.003)))^2 at the
+    initial state, with the block's day/night pattern (here each block of 50
+    is all day or all night when J is per cell: compare per-cell blocks)."""
+    o = oracle_cb6r5
+    n = 120
+    d = make_cells(cb6r5, np.arange(n), seed=12)
+    rtol, atol = 1e-3, 1e-7
+    e1 = o.stiffness(d["temp"], d["pres"], d["h2o"], d["rj"], d["y"], rtol, atol, blksize=1)
+    inew = o.inew2old() - 1
+    for c in range(0, n, 7):
+        rki, sun = o.rates(d["temp"][c:c + 1], d["pres"][c:c + 1], d["h2o"][c:c + 1], d["rj"][:, c:c + 1])
+        yn = d["y"][inew, c:c + 1]
+        ydot = np.asarray(o.feval(1 if sun else 2, rki, yn)).reshape(-1)
+        ref = np.sum((ydot / (yn[:, 0] + atol / min(rtol, 0.003))) ** 2)
+        assert abs(e1[c] - ref) <= 1e-12 * ref
+    # 50-cell blocks: a block with any sunlit cell uses the day pattern; the
+    # night pattern only drops photolysis terms that are zero at J = 0
+    e50 = o.stiffness(d["temp"], d["pres"], d["h2o"], d["rj"], d["y"], rtol, atol, blksize=50)
+    assert np.allclose(e50, e1, rtol=1e-12, atol=0)
