@@ -401,6 +401,15 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
         w(f"        dy[{sym.inew2old[i]} * n] = failed ? __builtin_nan(\"\") : LD(o + {i});")
     w("    }")
     w("}")
+    # the time-averaged state of the -Dsens build, mechanism order (rbsolver.F:592-596)
+    w("__device__ __noinline__ void store_yave(const SolveParams &P, int64_t cell, double *base_, double chemstep,")
+    w("                                        bool failed)")
+    w("{")
+    w("    ROS3_PROLOGUE")
+    w("    const int64_t n = P.stride;")
+    for i in range(ns):
+        w(f"    P.yave[{sym.inew2old[i]} * n + cell] = failed ? __builtin_nan(\"\") : LD(OFF_LAM + {i}) / chemstep;")
+    w("}")
     w("__device__ __noinline__ void tape_store(double *base_, double *rec, double h)")
     w("{")
     w("    ROS3_PROLOGUE")
@@ -435,6 +444,7 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w(f"    \"{variant}\", {budget}, {lookahead}, ros3::{cname}::WAVES_PER_EU, {1 if is_default else 0}, {refill_min},")
     w(f"    (const void *)&ros3::{cname}::solve_kernel_t<true>, ROS3_REVERSE_KERNEL,")
     w(f"    ros3::{cname}::TAPE_RS, (const void *)&ros3::{cname}::solve_kernel_t<false, true>,")
-    w(f"    ros3::{cname}::OFF_LAM}};   // rows of the workspace the forward kernel touches")
+    w(f"    ros3::{cname}::OFF_LAM,   // rows of the workspace the forward kernel touches")
+    w(f"    (const void *)&ros3::{cname}::solve_kernel_t<false, false, true>}};")
     w("#undef ROS3_REVERSE_KERNEL")
     return ("\n".join(out) + "\n", "\n".join(adj) + "\n"), sym

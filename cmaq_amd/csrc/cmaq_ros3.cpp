@@ -22,6 +22,7 @@
 #include "../../include/cmaq_ros3.h"
 #include "ros3_device.h"
 #include "ros3_chem_io.h"
+#include "ros3_ddm.h"
 
 using namespace ros3;
 
@@ -65,6 +66,7 @@ struct cmaq_ros3_ctx {
     int grid_blocks = 0;      // solver workgroups
     int grid_replay = 0, grid_reverse = 0;   // adjoint kernels' workgroups
     int grid_tlm = 0;                          // tangent-linear kernel's workgroups
+    int grid_yave = 0;                         // YAVE-accumulating forward's workgroups
     int num_cus = 0;
     int64_t waves = 0;
     // device buffers
@@ -88,6 +90,16 @@ struct cmaq_ros3_ctx {
     double fd_met[5] = {NAN, NAN, NAN, NAN, NAN};  // temp, pres, h2o, seawater, dens
     std::vector<double> fd_rj, fd_rki;
     unsigned char *scratch_dev = nullptr;          // this device's scratch array (device address)
+    // DDM-3D chemistry step: reaction tables in the solver's species order
+    DdmRxn *d_ddm_rx = nullptr;
+    int32_t *d_ddm_act = nullptr, *d_ddm_pid = nullptr, *d_ddm_inew2old = nullptr;
+    double *d_ddm_psc = nullptr;
+    int32_t ddm_nact = 0;
+    std::vector<DdmRxn> ddm_rx_host;               // for the host checks of rate parameters
+    double *d_yave = nullptr;                      // YAVE when the caller passes none
+    size_t yave_bytes = 0;
+    float *d_ddm_a = nullptr;                      // matrices of the workgroups when not in LDS
+    size_t ddm_a_bytes = 0;
     double *d_rki = nullptr;
     int64_t rki_cells = 0;
     // adjoint tapes
@@ -388,13 +400,14 @@ static int grid_for(cmaq_ros3_ctx *c, const void *kernel, int *grid)
 
 static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
 {
-    int grid = 0, grid_a = 0, grid_b = 0, grid_t = 0;
+    int grid = 0, grid_a = 0, grid_b = 0, grid_t = 0, grid_y = 0;
     if (int rc = grid_for(c, mk->solve_kernel, &grid)) return rc;
+    if (int rc = grid_for(c, mk->yave_kernel, &grid_y)) return rc;
     if (int rc = grid_for(c, mk->replay_kernel, &grid_a)) return rc;
     if (int rc = grid_for(c, mk->reverse_kernel, &grid_b)) return rc;
     if (int rc = grid_for(c, mk->tlm_kernel, &grid_t)) return rc;
     const int wpb = kSolveBlock / kWave;
-    const int64_t waves = (int64_t)std::max(std::max(grid, grid_t), std::max(grid_a, grid_b)) * wpb;
+    const int64_t waves = (int64_t)std::max(std::max(std::max(grid, grid_t), std::max(grid_a, grid_b)), grid_y) * wpb;
     // per-wave workspace stride: NWS rows of 64 doubles plus an optional pad
     // of rows that staggers the waves' bases across memory channels
     int64_t pad = 0;
@@ -414,6 +427,7 @@ static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
     c->grid_replay = grid_a;
     c->grid_reverse = grid_b;
     c->grid_tlm = grid_t;
+    c->grid_yave = grid_y;
     c->ws_stride = stride;
     c->waves = (int64_t)(c->ws_doubles / (size_t)stride);
     return 0;
@@ -456,6 +470,28 @@ extern "C" int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech, c
     c->nphotab = mech->nphotab;
     c->nhetero = mech->nhetero;
     if (decode_rates(t, c->ops, c->mul, why) != 0) { delete c; return set_err(nullptr, CMAQ_ROS3_E_MECH, why); }
+    // DDM-3D tables: every reaction's reactants and products in the solver's
+    // species order, and the active (NREACT > 0) list in reference order
+    std::vector<int32_t> ddm_act, ddm_pid;
+    std::vector<double> ddm_psc;
+    {
+        std::vector<int32_t> iold2new(c->ns);
+        for (int i = 0; i < c->ns; ++i) iold2new[mk->inew2old[i]] = i;
+        c->ddm_rx_host.assign(c->nr, DdmRxn{});
+        for (int r = 0; r < c->nr; ++r) {
+            DdmRxn &x = c->ddm_rx_host[r];
+            x.nre = mech->nreact[r];
+            x.npr = mech->nprdct[r];
+            x.poff = (int32_t)ddm_pid.size();
+            for (int q = 0; q < x.nre; ++q) x.re[q] = iold2new[t.irr(r, q) - 1];
+            for (int q = 0; q < x.npr; ++q) {
+                ddm_pid.push_back(iold2new[t.irr(r, 3 + q) - 1]);
+                ddm_psc.push_back(t.sc(r, q));
+            }
+            if (x.nre > 0) ddm_act.push_back(r);
+        }
+        c->ddm_nact = (int32_t)ddm_act.size();
+    }
     if ((int)c->mul.size() > kMaxMul) { delete c; return set_err(nullptr, CMAQ_ROS3_E_MECH, "too many multipliers"); }
     // tolerances in solver order (rbsolver.F:434-441 indexes them by the solver species)
     c->rtol_new.resize(c->ns);
@@ -517,6 +553,18 @@ extern "C" int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech, c
             INIT_TRY(hipMemcpy(c->d_mul, c->mul.data(), c->mul.size() * sizeof(RateMul), hipMemcpyHostToDevice));
         INIT_TRY(hipMemcpy(c->d_rtol, c->rtol_new.data(), c->ns * sizeof(double), hipMemcpyHostToDevice));
         INIT_TRY(hipMemcpy(c->d_atol, c->atol_new.data(), c->ns * sizeof(double), hipMemcpyHostToDevice));
+        if (dalloc(c, &c->d_ddm_rx, c->nr) || dalloc(c, &c->d_ddm_act, ddm_act.size()) ||
+            dalloc(c, &c->d_ddm_pid, ddm_pid.size()) || dalloc(c, &c->d_ddm_psc, ddm_psc.size()) ||
+            dalloc(c, &c->d_ddm_inew2old, c->ns))
+            goto fail_nomem;
+        INIT_TRY(hipMemcpy(c->d_ddm_rx, c->ddm_rx_host.data(), c->nr * sizeof(DdmRxn), hipMemcpyHostToDevice));
+        if (!ddm_act.empty())
+            INIT_TRY(hipMemcpy(c->d_ddm_act, ddm_act.data(), ddm_act.size() * 4, hipMemcpyHostToDevice));
+        if (!ddm_pid.empty()) {
+            INIT_TRY(hipMemcpy(c->d_ddm_pid, ddm_pid.data(), ddm_pid.size() * 4, hipMemcpyHostToDevice));
+            INIT_TRY(hipMemcpy(c->d_ddm_psc, ddm_psc.data(), ddm_psc.size() * 8, hipMemcpyHostToDevice));
+        }
+        INIT_TRY(hipMemcpy(c->d_ddm_inew2old, mk->inew2old, c->ns * 4, hipMemcpyHostToDevice));
     }
 #undef INIT_TRY
     *out = c;
@@ -535,6 +583,8 @@ extern "C" void cmaq_ros3_free(cmaq_ros3_ctx *c)
     hipFree(c->d_ops); hipFree(c->d_mul); hipFree(c->d_rtol); hipFree(c->d_atol);
     hipFree(c->d_ws); hipFree(c->d_small); hipFree(c->d_fail_detail); hipFree(c->d_rki); hipFree(c->d_io);
     hipFree(c->d_tape); hipFree(c->d_tape_off); hipFree(c->d_order); hipFree(c->d_sort);
+    hipFree(c->d_ddm_rx); hipFree(c->d_ddm_act); hipFree(c->d_ddm_pid); hipFree(c->d_ddm_inew2old);
+    hipFree(c->d_ddm_psc); hipFree(c->d_yave); hipFree(c->d_ddm_a);
     hipFree(c->d_step_na); hipFree(c->d_step_nr);
     hipFree(c->d_map); hipFree(c->d_block); hipFree(c->d_grid);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
@@ -873,6 +923,94 @@ extern "C" int cmaq_ros3_tlm_device(cmaq_ros3_ctx *c, int64_t n, double chemstep
     HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
     HIP_TRY(c, hipEventRecord(c->ev[0], s));
     return launch_solve(c, n, chemstep, rki, y, nacc, nrej, fail, s, dy, dk, ndir);
+}
+
+extern "C" int cmaq_ros3_solve_yave_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,
+                                           double *y, double *yave, int32_t *nacc, int32_t *nrej,
+                                           cmaq_ros3_fail *fail, void *stream)
+{
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (n > 0 && (!rki || !y || !yave)) return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
+    HIP_TRY(c, hipEventRecord(c->ev[0], s));
+    HIP_TRY(c, hipEventRecord(c->ev[1], s));
+    SolveParams P;
+    solve_params(c, P, n, chemstep, rki, y, nacc, nrej);
+    if (c->user_order && c->user_order_n == n) P.order = c->user_order;
+    P.yave = yave;
+    void *args[] = {&P};
+    HIP_TRY(c, hipLaunchKernel(c->mk->yave_kernel, dim3(c->grid_yave), dim3(kSolveBlock), args, 0, s));
+    HIP_TRY(c, hipEventRecord(c->ev[2], s));
+    unsigned char small[64];
+    HIP_TRY(c, hipMemcpyAsync(small, c->d_small, 64, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    float ms_s = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
+    c->rates_ms = 0.0;
+    c->solve_ms = ms_s;
+    const int rc = check_fail(c, small, fail);
+    note_fail_inputs(c, n, nullptr, nullptr, nullptr, nullptr, nullptr, rki);
+    return rc;
+}
+
+extern "C" int cmaq_ros3_ddm_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki, double *y,
+                                    double *yave, int32_t np, const int32_t *prate, float *sens, int32_t *nacc,
+                                    int32_t *nrej, cmaq_ros3_fail *fail, void *stream)
+{
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (np < 0 || np > kDdmMaxParams)
+        return set_err(c, CMAQ_ROS3_E_ARG, "np must be 0.." + std::to_string(kDdmMaxParams));
+    if (n > 0 && (!rki || !y || !nacc || (np > 0 && (!prate || !sens))))
+        return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    for (int p = 0; p < np; ++p) {
+        if (prate[p] < 0 || prate[p] > c->nr) return set_err(c, CMAQ_ROS3_E_ARG, "prate out of range");
+        if (prate[p] > 0 && c->ddm_rx_host[prate[p] - 1].nre == 0)
+            return set_err(c, CMAQ_ROS3_E_ARG, "rate parameter of a reaction without reactants");
+    }
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (!yave) {
+        if (int rc = grow_buffer(c, (void **)&c->d_yave, &c->yave_bytes,
+                                 (size_t)n * c->ns * sizeof(double), "YAVE"))
+            return rc;
+        yave = c->d_yave;
+    }
+    int rc = cmaq_ros3_solve_yave_device(c, n, chemstep, rki, y, yave, nacc, nrej, fail, stream);
+    if (rc != 0 && rc != CMAQ_ROS3_E_CONVERGENCE && rc != CMAQ_ROS3_E_STEPS) return rc;
+    if (np == 0) return rc;
+    // the matrix in LDS when it fits, else a slice of global memory per workgroup
+    const int ns = c->ns;
+    const size_t extra = ddm_lds_extra(ns, np), mat = (size_t)ns * ns * sizeof(float);
+    int max_lds = 0;
+    HIP_TRY(c, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    bool in_lds = mat + extra <= (size_t)max_lds &&
+                  hipFuncSetAttribute((const void *)ddm_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(mat + extra)) == hipSuccess;
+    (void)hipGetLastError();
+    const size_t lds = (in_lds ? mat : 0) + extra;
+    int per_cu = 0;
+    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)ddm_kernel, kDdmBlock, lds));
+    int64_t grid = std::min<int64_t>(n, (int64_t)c->num_cus * std::max(per_cu, 1));
+    if (!in_lds) {
+        grid = std::min<int64_t>(grid, 4 * (int64_t)c->num_cus);
+        if (int e = grow_buffer(c, (void **)&c->d_ddm_a, &c->ddm_a_bytes, (size_t)grid * mat, "DDM matrices"))
+            return e;
+    }
+    DdmParams D;
+    std::memset(&D, 0, sizeof D);
+    D.n = n; D.ns = ns; D.nact = c->ddm_nact; D.np = np; D.a_in_lds = in_lds ? 1 : 0;
+    D.rx = c->d_ddm_rx; D.act = c->d_ddm_act; D.pid = c->d_ddm_pid; D.psc = c->d_ddm_psc;
+    D.inew2old = c->d_ddm_inew2old;
+    for (int p = 0; p < np; ++p) D.prate[p] = prate[p];
+    D.chemstep = chemstep; D.rki = rki; D.yave = yave; D.nacc = nacc; D.sens = sens; D.ga = c->d_ddm_a;
+    hipLaunchKernelGGL(ddm_kernel, dim3((unsigned)grid), dim3(kDdmBlock), lds, s, D);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return rc;
 }
 
 extern "C" int cmaq_ros3_adjoint_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,

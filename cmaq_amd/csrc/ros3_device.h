@@ -269,6 +269,7 @@ struct SolveParams {
     int64_t *tape_head;       // [stride] last record of each cell (-1: none), or nullptr
     unsigned long long *tape_count;   // records taken from the arena so far
     int64_t tape_cap;         // arena capacity in records
+    double *yave;             // YAVE run: [ns][stride] time-averaged state, mechanism order
 };
 
 __device__ inline int lane_id() { return __lane_id(); }

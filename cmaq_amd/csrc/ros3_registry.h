@@ -21,6 +21,7 @@ struct MechKernels {
     int tape_rs;                      // doubles per tape record
     const void *tlm_kernel;           // solve_kernel carrying a tangent-linear direction
     int nws_fwd;                      // workspace rows (entries) the forward kernel touches
+    const void *yave_kernel;          // solve_kernel accumulating the time-averaged state
 };
 
 }  // namespace ros3

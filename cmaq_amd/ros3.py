@@ -97,6 +97,9 @@ def lib():
         L.cmaq_ros3_rates_device.argtypes = [vp, C.c_int64] + [vp] * 7 + [vp]
         L.cmaq_ros3_solve_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 4 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_solve_device_noalloc.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 4 + [C.POINTER(_Fail), vp]
+        L.cmaq_ros3_solve_yave_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 5 + [C.POINTER(_Fail), vp]
+        L.cmaq_ros3_ddm_device.argtypes = [vp, C.c_int64, C.c_double, vp, vp, vp, C.c_int32, _i32p, vp, vp, vp,
+                                           C.POINTER(_Fail), vp]
         L.cmaq_ros3_adjoint_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 5 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_last_adjoint_timing.argtypes = [vp, _f64p, _f64p, C.POINTER(C.c_int64)]
         L.cmaq_ros3_solve_tape_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 5 + [C.c_int64] + \
@@ -498,6 +501,51 @@ class Ros3Solver:
         f = _Fail()
         rc = lib().cmaq_ros3_solve_device_noalloc(self._ctx, n, chemstep, _ptr(rki), _ptr(y), _ptr(nacc),
                                                   _ptr(nrej), C.byref(f), C.c_void_p(stream))
+        fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, fail)
+        return fail if rc != 0 else None
+
+    def solve_yave_device(self, chemstep, rki, y, yave, nacc=None, nrej=None, stream=None, raise_on_fail=True):
+        """solve_device that also writes YAVE [ns, n], the time-averaged state
+        of the -Dsens build; see cmaq_ros3_solve_yave_device."""
+        import torch
+        n = y.numel() // self.ns
+        self._check([("rki", rki, self.nr), ("y", y, self.ns), ("yave", yave, self.ns), ("nacc", nacc, 1),
+                     ("nrej", nrej, 1)], n)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_ros3_solve_yave_device(self._ctx, n, chemstep, _ptr(rki), _ptr(y), _ptr(yave), _ptr(nacc),
+                                               _ptr(nrej), C.byref(f), C.c_void_p(stream))
+        fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, fail)
+        return fail if rc != 0 else None
+
+    DDM_MAX_PARAMS = 16
+
+    def ddm_device(self, chemstep, rki, y, sens, prate, nacc, yave=None, nrej=None, stream=None,
+                   raise_on_fail=True):
+        """One sync step with CMAQ-DDM-3D's chemistry sensitivity step: y
+        advanced in place; sens [np, ns, n] float32 (mechanism order) advanced
+        by one Crank-Nicolson step; prate[p] = 0 (propagation only) or the
+        1-based reaction whose ln k parameter p is.  See cmaq_ros3_ddm_device."""
+        import torch
+        n = y.numel() // self.ns
+        npar = len(prate)
+        self._check([("rki", rki, self.nr), ("y", y, self.ns), ("nacc", nacc, 1), ("nrej", nrej, 1),
+                     ("yave", yave, self.ns)], n)
+        if not (sens.is_cuda and sens.dtype == torch.float32 and sens.is_contiguous()
+                and sens.numel() == npar * self.ns * n):
+            raise ValueError("sens must be a contiguous float32 CUDA tensor of np x ns x n")
+        pr = np.ascontiguousarray(prate, np.int32)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_ros3_ddm_device(self._ctx, n, chemstep, _ptr(rki), _ptr(y), _ptr(yave), npar,
+                                        pr.ctypes.data_as(_i32p), _ptr(sens), _ptr(nacc), _ptr(nrej), C.byref(f),
+                                        C.c_void_p(stream))
         fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
         if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
             raise self._err(rc, fail)

@@ -285,6 +285,36 @@ int cmaq_ros3_tlm_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min
                          cmaq_ros3_fail *fail, void *stream);
 
 /*
+ * cmaq_ros3_solve_device that also returns the time-averaged state of the
+ * ros3 -Dsens build, YAVE = sum over accepted steps of 0.5 * (Y + MAX(YP,
+ * CONMIN)) * DT, divided by CHEMSTEP (rbsolver.F:341-343, 468-472,
+ * 592-596): yave [numb_mech_spc][ncells] out, mechanism order, NaN for a
+ * failing cell.  Same steps, y and counts as cmaq_ros3_solve_device.
+ */
+int cmaq_ros3_solve_yave_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep,
+                                const double *rki, double *y_ppm, double *yave,
+                                int32_t *n_accepted, int32_t *n_rejected, cmaq_ros3_fail *fail,
+                                void *stream);
+
+/*
+ * CMAQ-DDM-3D's chemistry step (SOLVE_DDM3D_CHEM, ddm3d/DDM3D_CHEM.F:155-511)
+ * after one sync step: cmaq_ros3_solve_yave_device (y advanced, yave out --
+ * NULL keeps it internal), then for each of np <= 16 sensitivity parameters
+ * one Crank-Nicolson step over the sync step with the REAL(4) Jacobian at
+ * YAVE, A S_new = (I + h/2 J) S_old + P, solved with LINPACK SGEFA / SGESL
+ * (ddm3d/s_lu.F) in the solver's species order as the reference does.
+ * prate [np] (host): 0 for a parameter whose chemistry step is propagation
+ * only (initial conditions, emissions, boundaries), r > 0 for the rate
+ * constant of reaction r (1-based; the sensitivity to ln k, IPT = 5).
+ * sens [np][numb_mech_spc][ncells] REAL(4), mechanism order, in/out (the
+ * cell-fastest slices of SENGRID); a failing cell gets NaN.  n_accepted is
+ * required.  Device memory, `stream` as in cmaq_ros3_run_device.
+ */
+int cmaq_ros3_ddm_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep, const double *rki,
+                         double *y_ppm, double *yave, int32_t np, const int32_t *prate, float *sens,
+                         int32_t *n_accepted, int32_t *n_rejected, cmaq_ros3_fail *fail, void *stream);
+
+/*
  * Scheduling hint for the forward calls (cmaq_ros3_run_device,
  * cmaq_ros3_solve_device and cmaq_ros3_run): cells are handed to lanes in
  * the order order[0..n-1] (a device array holding a permutation of

@@ -84,6 +84,8 @@ def lib():
         L.om_adjoint_run.argtypes = [vp, C.c_int64, C.c_double] + [_f64p] * 7 + [_i32p]
         L.om_ddm_cell.argtypes = [vp, C.c_double, _f64p, _f64p, _f64p, _f64p, C.c_int, _i32p,
                                   C.POINTER(C.c_float)]
+        L.om_ddm_step.argtypes = [vp, C.c_double, _f64p, _f64p, C.c_int, _i32p, C.POINTER(C.c_float)]
+        L.om_ddm_step.restype = None
         L.om_tlm_cell.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, C.c_int, _f64p, _f64p]
         _f32p = C.POINTER(C.c_float)
         L.om_gather.argtypes = [C.c_int] * 6 + [_i32p, _i32p, _f64p] + [_f32p] * 8 + [_f64p, C.c_int] + \
@@ -370,6 +372,23 @@ class Oracle:
             return s_old[:ns].astype(np.float64), s_old[ns:].astype(np.float64), yend
         wv = np.asarray(w, np.float64)
         return s_old[:ns].astype(np.float64) @ wv, s_old[ns:].astype(np.float64) @ wv, yend
+
+    def ddm_step(self, chemstep, rki, yave, sens, prate):
+        """The Crank-Nicolson sensitivity step alone (ddm3d/DDM3D_CHEM.F:296-395)
+        from a given time-averaged state yave (mechanism order): sens
+        [np, ns] float32 in mechanism order, prate[p] = 0 (initial-condition
+        parameter) or the 1-based reaction of a rate parameter.  Returns the
+        new sensitivities, mechanism order."""
+        inew = self.inew2old() - 1
+        ya = self._to_new(yave)
+        rk = np.ascontiguousarray(rki, np.float64)
+        s_new = np.ascontiguousarray(np.asarray(sens, np.float32)[:, inew])
+        pr = np.ascontiguousarray(prate, np.int32)
+        lib().om_ddm_step(self._ctx, chemstep, _p(rk, _f64p), _p(ya, _f64p), len(pr), _p(pr, _i32p),
+                          s_new.ctypes.data_as(C.POINTER(C.c_float)))
+        out = np.empty_like(s_new)
+        out[:, inew] = s_new
+        return out
 
 
 # ---- CHEM's CGRID gather / scatter (rbdriver.F:460-509, 577-595) ----

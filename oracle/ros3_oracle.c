@@ -1248,54 +1248,11 @@ static void sgesl(const float *a, int n, const int *ipvt, float *b)
     }
 }
 
-int om_ddm_cell(const om_ctx *c, double chemstep, const double *rki, double *y,
-                const double *rtol, const double *atol, int np, const int32_t *prate, float *sens)
+/* The Crank-Nicolson part alone, from a given YAVE (NEW order). */
+void om_ddm_step(const om_ctx *c, double chemstep, const double *rki, const double *yave, int np,
+                 const int32_t *prate, float *sens)
 {
     const int ns = c->ns, ncsp = 1;
-    const double rnspec = (double)(1.0f / (float)ns);
-    om_work w;
-    work_alloc(c, ncsp, &w);
-    double *yave = xcalloc(ns, 8);
-    double dt = DTMIN > DTSTART ? DTMIN : DTSTART, tnow = 0.0;
-    int lpass = 0, rc = 0;
-    long attempts = 0;
-    while (tnow < chemstep) {
-        if (++attempts > OM_MAX_ATTEMPTS) { rc = -2; break; }
-        double tend = tnow + dt;
-        if (tend > chemstep) { dt = chemstep - tnow; tend = chemstep; }
-        ros3_stages1(c, ncsp, rki, y, dt, w.ydot, w.cc2, w.vdiag, w.k1, w.k2, w.k3, w.yp, w.yn);
-        double err = 0.0;
-        for (int n = 0; n < ns; ++n) {
-            double ytol = atol[n] + rtol[n] * fabs(w.yn[n]);
-            double e = (D1 * w.k1[n] + D2 * w.k2[n] + D3 * w.k3[n]) / ytol;
-            err = err + e * e;
-        }
-        double s = sqrt(err * rnspec), maxerr = UROUND;
-        if (!(s <= maxerr)) maxerr = s;
-        double dtfac = 0.9 / pow(maxerr, GROW);
-        if (maxerr <= 1.0) {
-            for (int q = 0; q < ns; ++q) {
-                double yn = w.yn[q] > CONMIN ? w.yn[q] : CONMIN;
-                yave[q] = yave[q] + 0.5 * (y[q] + yn) * dt;
-                y[q] = yn;
-            }
-            tnow = tend;
-            double hi = lpass ? FACMAX : FACONE;
-            if (dtfac > hi) dtfac = hi;
-            if (dtfac < FACMIN) dtfac = FACMIN;
-            double dn = dtfac * dt;
-            if (dn < DTMIN) dn = DTMIN;
-            dt = dn < DTMAX ? dn : DTMAX;
-            lpass = 1;
-        } else {
-            dt = FACMIN * dt;
-            lpass = 0;
-            if (dt < DTMIN) { rc = -1; break; }
-        }
-    }
-    work_free(&w);
-    if (rc) { free(yave); return rc; }
-    for (int q = 0; q < ns; ++q) yave[q] = yave[q] / chemstep;
     /* Jacobian at YAVE accumulated in REAL(4) (MECHANISM_FUNCTIONS.F:831-869) */
     float *jac = xcalloc((size_t)ns * ns, 4), *a = xcalloc((size_t)ns * ns, 4), *a1 = xcalloc((size_t)ns * ns, 4);
     for (int nrx = 1; nrx <= c->nuserat[ncsp]; ++nrx) {
@@ -1349,9 +1306,62 @@ int om_ddm_cell(const om_ctx *c, double chemstep, const double *rki, double *y,
         sgesl(a, ns, ipvt, prd);
         for (int s = 0; s < ns; ++s) sp[s] = fabsf(prd[s]) < 1.0e-25f ? 0.0f : prd[s];
     }
-    free(yave); free(jac); free(a); free(a1); free(ipvt); free(sold); free(prd); free(prate_v);
+    free(jac); free(a); free(a1); free(ipvt); free(sold); free(prd); free(prate_v);
+}
+
+int om_ddm_cell(const om_ctx *c, double chemstep, const double *rki, double *y,
+                const double *rtol, const double *atol, int np, const int32_t *prate, float *sens)
+{
+    const int ns = c->ns, ncsp = 1;
+    const double rnspec = (double)(1.0f / (float)ns);
+    om_work w;
+    work_alloc(c, ncsp, &w);
+    double *yave = xcalloc(ns, 8);
+    double dt = DTMIN > DTSTART ? DTMIN : DTSTART, tnow = 0.0;
+    int lpass = 0, rc = 0;
+    long attempts = 0;
+    while (tnow < chemstep) {
+        if (++attempts > OM_MAX_ATTEMPTS) { rc = -2; break; }
+        double tend = tnow + dt;
+        if (tend > chemstep) { dt = chemstep - tnow; tend = chemstep; }
+        ros3_stages1(c, ncsp, rki, y, dt, w.ydot, w.cc2, w.vdiag, w.k1, w.k2, w.k3, w.yp, w.yn);
+        double err = 0.0;
+        for (int n = 0; n < ns; ++n) {
+            double ytol = atol[n] + rtol[n] * fabs(w.yn[n]);
+            double e = (D1 * w.k1[n] + D2 * w.k2[n] + D3 * w.k3[n]) / ytol;
+            err = err + e * e;
+        }
+        double s = sqrt(err * rnspec), maxerr = UROUND;
+        if (!(s <= maxerr)) maxerr = s;
+        double dtfac = 0.9 / pow(maxerr, GROW);
+        if (maxerr <= 1.0) {
+            for (int q = 0; q < ns; ++q) {
+                double yn = w.yn[q] > CONMIN ? w.yn[q] : CONMIN;
+                yave[q] = yave[q] + 0.5 * (y[q] + yn) * dt;
+                y[q] = yn;
+            }
+            tnow = tend;
+            double hi = lpass ? FACMAX : FACONE;
+            if (dtfac > hi) dtfac = hi;
+            if (dtfac < FACMIN) dtfac = FACMIN;
+            double dn = dtfac * dt;
+            if (dn < DTMIN) dn = DTMIN;
+            dt = dn < DTMAX ? dn : DTMAX;
+            lpass = 1;
+        } else {
+            dt = FACMIN * dt;
+            lpass = 0;
+            if (dt < DTMIN) { rc = -1; break; }
+        }
+    }
+    work_free(&w);
+    if (rc) { free(yave); return rc; }
+    for (int q = 0; q < ns; ++q) yave[q] = yave[q] / chemstep;
+    om_ddm_step(c, chemstep, rki, yave, np, prate, sens);
+    free(yave);
     return 0;
 }
+
 
 /* ---------------------------------------------------------------------------
  * CHEM's CGRID gather and scatter (rbdriver.F:460-509, 577-595).  Every

@@ -139,6 +139,10 @@ void om_tlm_cell(const om_ctx *c, int ncsp, const double *rki, const double *y0,
  * condition parameter (or prate = NULL). */
 int om_ddm_cell(const om_ctx *c, double chemstep, const double *rki, double *y,
                 const double *rtol, const double *atol, int np, const int32_t *prate, float *sens);
+/* The sensitivity step of om_ddm_cell alone, from a given YAVE (NEW order,
+ * already divided by chemstep). */
+void om_ddm_step(const om_ctx *c, double chemstep, const double *rki, const double *yave, int np,
+                 const int32_t *prate, float *sens);
 
 /* CHEM's CGRID gather (rbdriver.F:460-509) and scatter (rbdriver.F:577-595)
  * in the driver's REAL(4) arithmetic, for ncells = ncols*nrows*nlays cells in

@@ -118,8 +118,8 @@ def test_library_carries_device_code_of_every_kernel():
     """Each variant is compiled as two translation units (forward kernels and
     registry record; reverse kernel).  The library's gfx950 code objects must
     hold the kernel descriptor of every kernel the registry record points at:
-    the forward, tape-recording and tangent-linear instantiations of
-    solve_kernel_t and reverse_kernel."""
+    the forward, tape-recording, tangent-linear and YAVE-accumulating
+    instantiations of solve_kernel_t and reverse_kernel."""
     from cmaq_amd.build import VARIANTS, variant_name
     _lib()
     blob = open(ros3.LIB_PATH, "rb").read()
@@ -128,8 +128,8 @@ def test_library_carries_device_code_of_every_kernel():
         for (b, l, order) in variants:
             c = f"{name}_{variant_name(b, l, order)}"
             ns = f"_ZN4ros3{len(c)}{c}"
-            kds = [f"{ns}14solve_kernel_tILb{tape}ELb{tlm}EEEvNS_11SolveParamsE.kd"
-                   for tape, tlm in ((0, 0), (1, 0), (0, 1))]
+            kds = [f"{ns}14solve_kernel_tILb{tape}ELb{tlm}ELb{yave}EEEvNS_11SolveParamsE.kd"
+                   for tape, tlm, yave in ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1))]
             kds.append(f"{ns}14reverse_kernelENS_11SolveParamsE.kd")
             missing += [k for k in kds if k.encode() not in blob]
     assert not missing, missing
