@@ -196,6 +196,44 @@ def build(names=None, verbose=True, force=False):
     return LIB
 
 
+SAN_DIR = os.path.join(HERE, "_build", "san")
+SAN_LIB = os.path.join(SAN_DIR, "libcmaq_ros3_san.so")
+# host code only: the device code of the same translation unit is built as usual
+SAN_HOST_FLAGS = ["-g", "-Wno-unused-value", "-Xarch_host", "-fsanitize=address", "-Xarch_host",
+                  "-fsanitize=undefined", "-Xarch_host", "-fno-sanitize-recover=all", "-Xarch_host",
+                  "-fno-omit-frame-pointer"]
+
+
+def build_host_sanitized():
+    """The host runtime (cmaq_ros3.cpp) with AddressSanitizer and
+    UndefinedBehaviorSanitizer on its host code, linked with the objects of
+    the regular build, as _build/san/libcmaq_ros3_san.so.  Load it with
+    CMAQ_ROS3_LIB and LD_PRELOAD of clang's ASan runtime (asan_runtime())."""
+    os.makedirs(SAN_DIR, exist_ok=True)
+    objs = sorted(os.path.join(OBJ, f) for f in os.listdir(OBJ) if f.endswith(".o") and f != "cmaq_ros3.o")
+    if not objs:
+        raise RuntimeError("no kernel objects: run the regular build first")
+    obj = _compile(os.path.join(CSRC, "cmaq_ros3.cpp"), os.path.join(SAN_DIR, "cmaq_ros3_san.o"),
+                   extra=SAN_HOST_FLAGS)
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-Xarch_host", "-fsanitize=address",
+           "-Xarch_host", "-fsanitize=undefined", "-o", SAN_LIB + ".tmp", obj] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(SAN_LIB + ".tmp", SAN_LIB)
+    return SAN_LIB
+
+
+def asan_runtime():
+    """clang's shared ASan runtime of this ROCm (the one the sanitized host
+    objects were compiled against), or None."""
+    import glob
+    hits = sorted(glob.glob(os.path.join(os.path.dirname(os.path.realpath(HIPCC)), "..", "lib", "llvm", "lib",
+                                         "clang", "*", "lib", "linux", "libclang_rt.asan-x86_64.so")))
+    hits += sorted(glob.glob("/opt/rocm/lib/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so"))
+    return hits[0] if hits else None
+
+
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if a != "--force"]
     build(args or None, force="--force" in sys.argv[1:])

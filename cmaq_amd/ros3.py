@@ -18,7 +18,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcmaq_ros3.so")
+# CMAQ_ROS3_LIB names another build of the same library (the host-sanitized
+# one in the CPU sanitizer test)
+LIB_PATH = os.environ.get("CMAQ_ROS3_LIB") or os.path.join(HERE, "libcmaq_ros3.so")
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)

@@ -13,7 +13,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle_ros3.so")
+# CMAQ_ORACLE_LIB names another build (the ASan/UBSan one of `make -C oracle san`)
+_LIB_PATH = os.environ.get("CMAQ_ORACLE_LIB") or os.path.join(_HERE, "liboracle_ros3.so")
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
