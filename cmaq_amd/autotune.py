@@ -16,7 +16,7 @@ import sys
 
 
 def tune(mech_name, ncells, workload="regional", seed=1, refills=(1, 2, 4), log=print, svoc_ppb=None,
-         waves_per_cu=(0,)):
+         waves_per_cu=(0,), variants=None):
     import torch
     from .inputs import make_cells_torch
     from .ros3 import Ros3Solver, load_mechanism
@@ -29,7 +29,8 @@ def tune(mech_name, ncells, workload="regional", seed=1, refills=(1, 2, 4), log=
     nj = torch.zeros(ncells, dtype=torch.int32, device=dev)
     results = []
     base = Ros3Solver(mech)
-    for var, rf, wpc in [(v, r, w) for v in base.variants() for r in refills for w in waves_per_cu]:
+    names = [v for v in base.variants() if not variants or v in variants]
+    for var, rf, wpc in [(v, r, w) for v in names for r in refills for w in waves_per_cu]:
         s = Ros3Solver(mech, refill_min=rf, waves_per_cu=wpc)
         s.set_variant(var)
         best = None
@@ -58,6 +59,8 @@ def main():
     p.add_argument("--out", default="tuning.json")
     p.add_argument("--workload", default="regional", choices=["regional", "clean", "mixed"])
     p.add_argument("--svoc", action="store_true", help="VOC/SVOC species log-uniform on [1e-5, 5] ppb")
+    p.add_argument("--refill", type=int, nargs="*", default=[1, 2, 4], help="refill thresholds to sweep")
+    p.add_argument("--variant", nargs="*", default=None, help="time only these built variants")
     p.add_argument("--waves-per-cu", type=int, nargs="*", default=[0],
                    help="resident-wave caps to sweep (0 = as many as the registers allow)")
     a = p.parse_args()
@@ -67,7 +70,8 @@ def main():
             table = json.load(f)
     for m in a.mech or ["cb6r5_ae7_aq"]:
         table[m] = tune(m, a.ncells, workload=a.workload, svoc_ppb=(1e-5, 5.0) if a.svoc else None,
-                        waves_per_cu=tuple(a.waves_per_cu), log=lambda s: print(s, file=sys.stderr, flush=True))
+                        refills=tuple(a.refill), waves_per_cu=tuple(a.waves_per_cu), variants=a.variant,
+                        log=lambda s: print(s, file=sys.stderr, flush=True))
         with open(a.out, "w") as f:
             json.dump(table, f, indent=1)
     print(json.dumps({m: {k: v for k, v in t.items() if k != "results"} for m, t in table.items()}))

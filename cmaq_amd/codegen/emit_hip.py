@@ -36,7 +36,7 @@ import struct
 from .programs import (adj_a_ops, adj_b_ops, adj_seed_ops, adj_u1_ops, adj_u2_ops, feval_ops,
                        jac_lu_ops, msolve_ops, solve_ops, tlm_rhs1_ops, tlm_rhs_ops, tlm_stage_ops,
                        tlm_update_ops, tsolve_ops, yp_ops)
-from .regalloc import Op, allocate
+from .regalloc import Op, allocate as _allocate
 from .symbolic import analyse
 
 # register budget (doubles per lane held by a generated function) and load
@@ -189,9 +189,14 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     # "+ldsN": N doubles per lane of LDS as a second tier for the forward
     # functions (clean copies of evicted values that are read again)
     lds = next((int(f[3:]) for f in flags if f.startswith("lds")), 0)
+    # "+fN": prefetch as far ahead (at most `lookahead` operations) as keeps N
+    # loads outstanding, instead of a fixed distance in operations: the same
+    # requests in flight with fewer registers held by prefetched values
+    inflight = next((int(f[1:]) for f in flags if f[:1] == "f" and f[1:].isdigit()), 0)
     # "+sbN": a scheduling barrier every N statements (see _function)
     sched = next((int(f[2:]) for f in flags if f.startswith("sb")), 0)
     fn = functools.partial(_function, sched=sched)
+    allocate = functools.partial(_allocate, inflight=inflight)
 
     def stage1_backward():
         # YP = Y + K1 (rbsolver.F:395-398) as soon as K1(i) is final

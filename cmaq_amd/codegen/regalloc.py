@@ -48,7 +48,7 @@ def _reads(op):
             if isinstance(a, tuple) and (k > 0 or op.rmw)]
 
 
-def allocate(ops, outputs, budget, lookahead, defined_on_entry, lds_slots=0, dead=frozenset()):
+def allocate(ops, outputs, budget, lookahead, defined_on_entry, lds_slots=0, dead=frozenset(), inflight=0):
     """Returns (items, counts) with counts of each item kind.
 
     outputs: array names whose final values must be in memory at the end,
@@ -167,9 +167,14 @@ def allocate(ops, outputs, budget, lookahead, defined_on_entry, lds_slots=0, dea
             lds_slot[q] = None
 
     pf = 0
+    pending = []      # use times of prefetched loads not yet used (inflight > 0)
     for t in range(n):
-        # prefetch the operands of the next `lookahead` operations
-        while pf < n and pf <= t + lookahead:
+        # prefetch the operands of the next `lookahead` operations; with
+        # inflight > 0, as far ahead as keeps `inflight` prefetched loads
+        # outstanding (at most `lookahead` operations)
+        if inflight:
+            pending = [u for u in pending if u > t]
+        while pf < n and pf <= t + lookahead and (not inflight or len(pending) < inflight):
             window, written = set(), set()
             for q in range(t, pf + 1):
                 window.update(_reads(ops[q]))
@@ -179,8 +184,11 @@ def allocate(ops, outputs, budget, lookahead, defined_on_entry, lds_slots=0, dea
             # prefetches never pin more than R - RESERVE registers
             if sum(1 for r in range(R) if pinned[r] > t) + len(_reads(ops[pf])) > R - reserve:
                 break
+            nload = counts["load"]
             if not all(ensure(s, t, window, pf) for s in _reads(ops[pf]) if s not in written):
                 break
+            if inflight and pf > t:
+                pending += [pf] * (counts["load"] - nload)
             pf += 1
         op = ops[t]
         need = _reads(op)

@@ -37,6 +37,20 @@ class _Tables(C.Structure):
     ]
 
 
+class _GearTol(C.Structure):
+    _fields_ = [("rtol", C.c_float), ("max_atol", C.c_float), ("min_atol", C.c_float), ("conc_floor", C.c_float)]
+
+
+class _GearStats(C.Structure):
+    _fields_ = [(k, C.c_long) for k in ("nsteps", "nsubfun", "npderiv", "numnewt", "ncfail", "nefail",
+                                         "numbkups", "nylowred")] + [("mxordused", C.c_int)]
+
+
+# RUNTIME_VARS.F:173-176 defaults (REAL(4)): GEAR_RTOL, GEAR_MAX_ATOL,
+# GEAR_MIN_ATOL, GEAR_CONC_FLOOR
+GEAR_TOL_DEFAULT = (1.0e-3, 1.0e-7, 1.0e-9, 1.0e-12)
+
+
 class _Fail(C.Structure):
     _fields_ = [("cell", C.c_int64), ("maxerr", C.c_double), ("dt", C.c_double), ("tnow", C.c_double),
                 ("dump", C.c_void_p)]
@@ -88,6 +102,18 @@ def lib():
         L.om_ddm_step.argtypes = [vp, C.c_double, _f64p, _f64p, C.c_int, _i32p, C.POINTER(C.c_float)]
         L.om_ddm_step.restype = None
         L.om_tlm_cell.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, C.c_int, _f64p, _f64p]
+        L.om_gear_coefficients.argtypes = [_f64p] * 4
+        L.om_gear_coefficients.restype = None
+        L.om_subfun.argtypes = [vp, C.c_int, C.c_int, _f64p, _f64p, _f64p]
+        L.om_subfun.restype = None
+        L.om_pderiv.argtypes = [vp, C.c_int, C.c_int, _f64p, _f64p, C.c_double, _f64p]
+        L.om_pderiv.restype = None
+        L.om_optimal_atol.argtypes = [vp, C.c_int, _f64p, C.POINTER(_GearTol), _f64p]
+        L.om_optimal_atol.restype = None
+        L.om_smvgear.argtypes = [vp, C.c_int, C.c_int, C.c_double, _f64p, _f64p, C.POINTER(_GearTol), C.c_int,
+                                 C.POINTER(_GearStats)]
+        L.om_gear_run.argtypes = [vp, C.c_int64, C.c_int, C.c_double] + [_f64p] * 7 + \
+            [C.POINTER(_GearTol), _i32p, _i32p, _f64p]
         _f32p = C.POINTER(C.c_float)
         L.om_gather.argtypes = [C.c_int] * 6 + [_i32p, _i32p, _f64p] + [_f32p] * 8 + [_f64p, C.c_int] + \
             [_f64p] * 7
@@ -271,6 +297,64 @@ class Oracle:
                            _p(args[2], _f64p), _p(het, _f64p), _p(sw, _f64p), _p(args[3], _f64p),
                            _p(rt, _f64p), _p(at, _f64p), _p(out, _f64p))
         return out
+
+    # ---- SMVGEAR (CCTM/src/gas/smvgear; gear_oracle.c) ----
+    @staticmethod
+    def _gear_tol(tol):
+        t = _GearTol()
+        t.rtol, t.max_atol, t.min_atol, t.conc_floor = tol or GEAR_TOL_DEFAULT
+        return t
+
+    def subfun(self, ncsp, rki, c_new):
+        rki = np.ascontiguousarray(rki, np.float64)
+        c_new = np.ascontiguousarray(c_new, np.float64)
+        g = np.zeros_like(c_new)
+        lib().om_subfun(self._ctx, ncsp, c_new.shape[1], _p(rki, _f64p), _p(c_new, _f64p), _p(g, _f64p))
+        return g
+
+    def pderiv(self, ncsp, rki, c_new, r1delt):
+        rki = np.ascontiguousarray(rki, np.float64)
+        c_new = np.ascontiguousarray(c_new, np.float64)
+        cc2 = np.zeros((self.iarray(ncsp) + 1, c_new.shape[1]))
+        lib().om_pderiv(self._ctx, ncsp, c_new.shape[1], _p(rki, _f64p), _p(c_new, _f64p), r1delt,
+                        _p(cc2, _f64p))
+        return cc2
+
+    def optimal_atol(self, c_new, tol=None):
+        c_new = np.ascontiguousarray(c_new, np.float64)
+        out = np.zeros(c_new.shape[1])
+        t = self._gear_tol(tol)
+        lib().om_optimal_atol(self._ctx, c_new.shape[1], _p(c_new, _f64p), C.byref(t), _p(out, _f64p))
+        return out
+
+    def smvgear(self, ncsp, chemstep, rki, y_new, sunlit, tol=None):
+        """SMVGEAR on one block (y in NEW order, copied).  Returns (rc, y,
+        stats dict)."""
+        rki = np.ascontiguousarray(rki, np.float64)
+        y = np.array(y_new, np.float64, copy=True, order="C")
+        t, st = self._gear_tol(tol), _GearStats()
+        rc = lib().om_smvgear(self._ctx, ncsp, y.shape[1], chemstep, _p(rki, _f64p), _p(y, _f64p), C.byref(t),
+                              int(bool(sunlit)), C.byref(st))
+        return rc, y, {k: getattr(st, k) for k, _ in _GearStats._fields_}
+
+    def gear_run(self, chemstep, temp, pres_atm, h2o_ppm, rj, y, het=None, seawater=None, blksize=1, tol=None,
+                 rki=None):
+        """The SMVGEAR block driver over every cell; y [ns, ncells] mechanism
+        order (copied).  rki [nr, ncells] replaces CALC_RCONST if given."""
+        temp = np.ascontiguousarray(temp, np.float64)
+        nc = temp.size
+        args = [np.ascontiguousarray(a, np.float64) for a in (pres_atm, h2o_ppm, rj)]
+        het = np.zeros((max(self.nhetero, 1), nc)) if het is None else np.ascontiguousarray(het, np.float64)
+        sw = np.zeros(nc) if seawater is None else np.ascontiguousarray(seawater, np.float64)
+        yy = np.array(y, np.float64, copy=True, order="C")
+        rk = None if rki is None else np.ascontiguousarray(rki, np.float64)
+        ns_, nf = np.zeros(nc, np.int32), np.zeros(nc, np.int32)
+        t = self._gear_tol(tol)
+        rc = lib().om_gear_run(self._ctx, nc, blksize, chemstep, _p(temp, _f64p), _p(args[0], _f64p),
+                               _p(args[1], _f64p), _p(args[2], _f64p), _p(het, _f64p), _p(sw, _f64p),
+                               _p(yy, _f64p), C.byref(t), _p(ns_, _i32p), _p(nf, _i32p),
+                               None if rk is None else _p(rk, _f64p))
+        return {"rc": rc, "y": yy, "nsteps": ns_, "nfail": nf}
 
     # ---- discrete adjoint with the accepted steps held fixed (one cell) ----
     # y / lam arguments here are in MECHANISM order; the day pattern is used.

@@ -161,6 +161,41 @@ void om_scatter(int ns, int ncols, int nrows, int nlays, const int32_t *cgrid_in
                 const int32_t *convert_conc, const double *species_molwt, const float *dens,
                 const double *y, float *cgrid);
 
+/* ---- SMVGEAR (CCTM/src/gas/smvgear; gear_oracle.c) ---------------------- */
+
+/* The Gear runtime tolerances, REAL(4) as RUNTIME_VARS.F:173-176 holds them
+ * (defaults 1e-3, 1e-7, 1e-9, 1e-12). */
+typedef struct om_gear_tol {
+    float rtol, max_atol, min_atol, conc_floor;
+} om_gear_tol;
+
+typedef struct om_gear_stats {
+    long nsteps, nsubfun, npderiv, numnewt, ncfail, nefail, numbkups, nylowred;
+    int mxordused;
+} om_gear_stats;
+
+/* ASET (6 x 7), squared PERTST (7 x 3), ENQQ1..3 (7 x 3), CONPST / CONP15
+ * (7 x 2) as JSPARSE leaves them (grsprse.F:675-713), row-major by order. */
+void om_gear_coefficients(double *aset, double *pertst, double *enqq, double *conpst);
+/* grsubfun.F (no emissions): gloss [ischan][nc] from cnew, NEW order. */
+void om_subfun(const om_ctx *c, int ncsp, int nc, const double *rki, const double *cnew, double *gloss);
+/* grpderiv.F: cc2 = I + r1delt * J(max-with-ZBOUND cnew); [iarray+1][nc]. */
+void om_pderiv(const om_ctx *c, int ncsp, int nc, const double *rki, const double *cnew, double r1delt,
+               double *cc2);
+/* GRVARS.F OPTIMAL_ATOL_PPM: per-cell absolute tolerance YLOWA. */
+void om_optimal_atol(const om_ctx *c, int nc, const double *cnew, const om_gear_tol *tol, double *ylowa);
+/* grsmvgear.F on one block of nc cells (NEW order y in/out).  Returns 0,
+ * -1 (YLOWA lowered MXLOWRED times), -2 (100 restarts) or -3 (step guard). */
+int om_smvgear(const om_ctx *c, int ncsp, int nc, double chemstep, const double *rki, double *y,
+               const om_gear_tol *tol, int sunlit, om_gear_stats *st);
+/* grdriver.F block loop: y [ns][ncells] MECHANISM order in/out; per-cell
+ * nsteps (-1 = failed block) and nfail; rki [nr][ncells] replaces the
+ * block's CALC_RCONST when not NULL (LSUNLIGHT still from rj). */
+int om_gear_run(const om_ctx *c, int64_t ncells, int blksize, double chemstep, const double *temp,
+                const double *pres, const double *h2o, const double *rj, const double *het,
+                const double *seawater, double *y, const om_gear_tol *tol, int32_t *nsteps,
+                int32_t *nfail, const double *rki);
+
 #ifdef __cplusplus
 }
 #endif
