@@ -31,7 +31,12 @@ The JSON line also carries
                  forward (details under "adjoint"; the mode that recomputes
                  the stages per sync step under "adjoint.replay_mode");
   adjoint_larger_mechanism - the same for CRACMM2 on 2,097,152 mixed cells
-                 with VOC/SVOC species, stages recomputed per sync step.
+                 with VOC/SVOC species, stages recomputed per sync step;
+  smvgear      - CMAQ's other sparse solver, SMVGEAR (variable-order Gear
+                 BDF), on 1,048,576 regional cells, one 5-min sync step:
+                 successful Gear cell-steps/s, its time against ROS3 on the
+                 same cells, and the oracle's SMVGEAR on a sample of them on
+                 the host cores.
 """
 
 import argparse
@@ -73,6 +78,11 @@ def _args():
     p.add_argument("--no-chem", action="store_true", help="skip the CHEM gather/scatter leg")
     p.add_argument("--big-adj-mech", default="cracmm2")
     p.add_argument("--big-adj-cells", type=int, default=2097152)
+    p.add_argument("--gear-cells", type=int, default=1048576)
+    p.add_argument("--no-gear", action="store_true", help="skip the SMVGEAR leg")
+    p.add_argument("--gear-tail-after", type=int, default=None,
+                   help="attempted Gear steps before a cell moves to the one-cell-per-wavefront kernel "
+                        "(cmaq_gear_set_tail; default: the library's)")
     p.add_argument("--no-big-adj", action="store_true",
                    help="skip the larger-mechanism forward + adjoint leg (CRACMM2, replay mode)")
     return p.parse_args()
@@ -407,6 +417,88 @@ def big_adjoint_leg(args, dev):
             "cells_without_gradient": bad}
 
 
+def _gear_cpu_worker(job):
+    """Times the oracle's SMVGEAR on one slice of cells with per-cell control
+    (blocks of one cell, the GPU kernel's computation; runs in a child
+    process)."""
+    mech_name, cells, seed, workload, nlay_cells = job
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from cmaq_amd.inputs import make_cells
+    from cmaq_amd.ros3 import load_mechanism
+    from oracle.oracle import Oracle
+    mech = load_mechanism(mech_name)
+    o = Oracle(mech)
+    d = make_cells(mech, np.asarray(cells), seed=seed, workload=workload, nlay_cells=nlay_cells)
+    t0 = time.perf_counter()
+    r = o.gear_run(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], d["y"], blksize=1)
+    dt = time.perf_counter() - t0
+    return dt, int(np.clip(r["nsteps"], 0, None).sum()), r["rc"]
+
+
+def gear_leg(args, mech, dev, workers):
+    """SMVGEAR (cmaq_gear_run_device) and ROS3 on the same regional cells, one
+    5-min sync step each, and the oracle's SMVGEAR on a bounded sample of them
+    with per-cell control on the host cores."""
+    import torch
+    from cmaq_amd.inputs import make_cells_torch
+    from cmaq_amd.ros3 import Ros3Solver
+    n = args.gear_cells
+    nlay_cells = max(1, n // 35)
+    d = make_cells_torch(mech, n, seed=args.seed + 6, workload="regional", nlay_cells=nlay_cells, device=dev)
+    s = Ros3Solver(mech)
+    if args.gear_tail_after is not None:
+        s.set_gear_tail(args.gear_tail_after, 0)
+    y = torch.empty_like(d["y"])
+    ns_ = torch.zeros(n, dtype=torch.int32, device=dev)
+    nf = torch.zeros(n, dtype=torch.int32, device=dev)
+    best = None
+    for k in range(2):
+        y.copy_(d["y"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.gear_run_device(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], y, ns_, nf, raise_on_fail=False)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0
+        best = ms if best is None else min(best, ms)
+        print(f"[bench] SMVGEAR call {k + 1}/2: {ms:.0f} ms, {s.query('gear_tail_cells_last')} long cells finished "
+              f"one per wavefront", file=sys.stderr, flush=True)
+    gear_ms = s.gear_last_timing()[1]
+    steps = ns_.clamp(min=0).to(torch.int64)
+    nsteps_total = int(steps.sum())
+    attempts_max = int((ns_.clamp(min=0) + nf).max())
+    q = torch.quantile(steps.to(torch.float64), torch.tensor([0.01, 0.5, 0.99], dtype=torch.float64, device=dev))
+    y.copy_(d["y"])
+    na = torch.zeros(n, dtype=torch.int32, device=dev)
+    nj = torch.zeros(n, dtype=torch.int32, device=dev)
+    s.run_device(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], y, na, nj, raise_on_fail=False)
+    torch.cuda.synchronize()
+    ros3_ms = s.last_timing()[1]
+    out = {"workload": f"{args.mech} SMVGEAR, regional, {n} cells, one 5-min sync step, GEAR_RTOL 1e-3, "
+                       f"GEAR_MAX/MIN_ATOL 1e-7/1e-9 ppm",
+           "ncells": n, "ms": best, "gear_kernel_ms": gear_ms,
+           "successful_cell_steps_per_s": nsteps_total / (gear_ms / 1000.0),
+           "steps_per_cell_p01_p50_p99": [float(v) for v in q.cpu()],
+           "failed_cells": int((ns_ < 0).sum()), "attempted_steps_longest_cell": attempts_max,
+           "long_cells_one_per_wavefront": int(s.query("gear_tail_cells_last")),
+           "ros3_kernel_ms_same_cells": ros3_ms, "gear_over_ros3_time": gear_ms / ros3_ms}
+    s.close()
+    if not args.no_cpu:
+        per_worker = 600
+        idx = np.linspace(0, n - 1, per_worker * workers).astype(np.int64)
+        jobs = [(args.mech, idx[w::workers].tolist(), args.seed + 6, "regional", nlay_cells) for w in range(workers)]
+        import multiprocessing as mp
+        t0 = time.perf_counter()
+        with mp.get_context("spawn").Pool(workers) as pool:
+            res = pool.map(_gear_cpu_worker, jobs)
+        busy = max(r[0] for r in res)
+        out["cpu_baseline"] = {"value": sum(r[1] for r in res) / busy, "unit": "successful cell-steps/s",
+                               "cores": workers, "kind": "port",
+                               "sample": f"{per_worker * workers} cells evenly spaced over the {n} cells, "
+                                         f"{workers} processes, per-cell control; slowest process {busy:.1f} s",
+                               "failed_blocks": int(sum(1 for r in res if r[2] != 0))}
+    return out
+
+
 def chem_io_leg(args, mech, solver, d, dims, dev):
     """CHEM-level entry points on the headline grid: the CGRID gather and
     scatter kernels (HBM-bound copies with unit conversion) and one whole
@@ -604,6 +696,14 @@ def main():
     if not args.no_big_adj:
         print("[bench] larger-mechanism adjoint leg", file=sys.stderr, flush=True)
         out["adjoint_larger_mechanism"] = big_adjoint_leg(args, dev)
+        torch.cuda.empty_cache()
+    if not args.no_gear:
+        print("[bench] SMVGEAR leg", file=sys.stderr, flush=True)
+        try:
+            avail = len(os.sched_getaffinity(0))
+        except AttributeError:
+            avail = os.cpu_count() or 1
+        out["smvgear"] = gear_leg(args, mech, dev, args.cpu_workers or min(16, avail))
         torch.cuda.empty_cache()
     wpc = max(1, solver.query("solve_waves") // max(1, torch.cuda.get_device_properties(dev).multi_processor_count))
     ceiling = row_stream_ceiling(wpc)

@@ -10,7 +10,7 @@ import os
 import subprocess
 import sys
 
-from .codegen.emit_hip import emit
+from .codegen.emit_hip import emit, emit_gear
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -91,41 +91,66 @@ def _write_if_changed(path, text):
     return True
 
 
-def tuned(name):
-    """(default variant, its refill threshold) for a mechanism."""
-    table = {}
+def _tuning_table():
     if os.path.exists(TUNING):
         with open(TUNING) as f:
-            table = json.load(f)
-    entry = table.get(name, {})
-    built = [variant_name(*v) for v in VARIANTS[name]]
+            return json.load(f)
+    return {}
+
+
+def variants_of(name):
+    """The variants built for a mechanism: the hand-kept table above plus the
+    knobs the autotuner chose (tuning.json "knobs": register budget,
+    lookahead, LU schedule), which need not be in the table."""
+    vs = list(VARIANTS[name])
+    knobs = _tuning_table().get(name, {}).get("knobs")
+    if knobs:
+        k = (int(knobs[0]), int(knobs[1]), str(knobs[2]))
+        if k not in vs:
+            vs.insert(0, k)
+    return vs
+
+
+def tuned(name):
+    """(default variant, its refill threshold) for a mechanism."""
+    entry = _tuning_table().get(name, {})
+    built = [variant_name(*v) for v in variants_of(name)]
     var = entry.get("variant")
     if var not in built:       # a variant no longer generated: the table's first
         var = built[0]
     return var, int(entry.get("refill_min", 1))
 
 
-def generate(names=None):
-    os.makedirs(GEN, exist_ok=True)
+def generate(names=None, variants=None, gen=GEN, obj=OBJ, gear=True):
+    """Writes the generated sources; `variants` (mechanism -> knob list)
+    replaces the built set (the autotuner's candidate library)."""
+    os.makedirs(gen, exist_ok=True)
     names = names or list(MECHS)
     srcs, cnames = [], []
     for name in names:
         mech = load_mech(name)
         dflt, refill = tuned(name)
-        for (b, l, order) in VARIANTS[name]:
+        for (b, l, order) in (variants[name] if variants else variants_of(name)):
             var = variant_name(b, l, order)
             cname = f"{name}_{var}"
+            units = []
+            gear_nws = 0
+            if var == dflt and gear:
+                # the SMVGEAR kernel, once per mechanism, on the default's schedule
+                text_gear, gear_nws = emit_gear(mech, cname, budget=b, lookahead=l, jac_order=order)
+                units.append(("_gear", text_gear))
             (text, text_adj), _ = emit(mech, cname, budget=b, lookahead=l, variant=var,
-                                       is_default=(var == dflt), refill_min=refill, jac_order=order)
-            for suffix, t in (("", text), ("_adj", text_adj)):
-                path = os.path.join(GEN, f"ros3_{cname}{suffix}.hip")
+                                       is_default=(var == dflt), refill_min=refill, jac_order=order,
+                                       gear_nws=gear_nws)
+            for suffix, t in [("", text), ("_adj", text_adj)] + units:
+                path = os.path.join(gen, f"ros3_{cname}{suffix}.hip")
                 _write_if_changed(path, t)
                 srcs.append((path, KERNEL_FLAGS))
             cnames.append(cname)
-    if set(names) == set(MECHS):
+    if set(names) == set(MECHS) or variants:
         # drop sources and objects of variants no longer in the table
-        keep = {f"ros3_{c}" for c in cnames} | {f"ros3_{c}_adj" for c in cnames}
-        for d, ext in ((GEN, ".hip"), (OBJ, ".o")):
+        keep = {f"ros3_{c}{u}" for c in cnames for u in ("", "_adj", "_gear")}
+        for d, ext in ((gen, ".hip"), (obj, ".o")):
             for f in os.listdir(d) if os.path.isdir(d) else []:
                 stem = f[:f.index(".")] if "." in f else f
                 if f.startswith("ros3_") and f.endswith((ext, ext + ".cmd")) and stem not in keep:
@@ -137,7 +162,7 @@ def generate(names=None):
     reg += [f"    &ros3_kernels_{name}," for name in cnames]
     reg.append("};")
     reg.append(f"extern \"C\" const int ros3_mech_registry_size = {len(cnames)};")
-    path = os.path.join(GEN, "registry.cpp")
+    path = os.path.join(gen, "registry.cpp")
     _write_if_changed(path, "\n".join(reg) + "\n")
     srcs.append((path, []))
     return srcs
@@ -152,8 +177,10 @@ def _compile(src, obj, host_only=False, extra=(), force=False):
     # with the same command
     stamp = obj + ".cmd"
     same_cmd = os.path.exists(stamp) and open(stamp).read() == " ".join(cmd)
-    deps = [src, os.path.join(CSRC, "ros3_device.h"), os.path.join(CSRC, "ros3_registry.h"), os.path.join(CSRC, "ros3_solver_kernel.inc"), os.path.join(CSRC, "ros3_adjoint_kernel.inc"),
-            os.path.join(CSRC, "ros3_registry.h"), os.path.join(HERE, "..", "include", "cmaq_ros3.h")]
+    deps = [src] + [os.path.join(CSRC, f) for f in ("ros3_device.h", "ros3_registry.h", "ros3_solver_kernel.inc",
+                                                    "ros3_adjoint_kernel.inc", "ros3_gear.h",
+                                                    "ros3_gear_kernel.inc", "ros3_pow.h")] + \
+        [os.path.join(HERE, "..", "include", "cmaq_ros3.h")]
     if src.endswith("cmaq_ros3.cpp"):
         deps.append(os.path.join(CSRC, "ros3_chem_io.h"))
     if not force and same_cmd and os.path.exists(obj) and \
@@ -167,25 +194,31 @@ def _compile(src, obj, host_only=False, extra=(), force=False):
     return obj
 
 
-def build(names=None, verbose=True, force=False):
-    """Generate and compile; force=True recompiles every object."""
-    os.makedirs(OBJ, exist_ok=True)
-    gen_srcs = generate(names)
+def build(names=None, verbose=True, force=False, variants=None, lib=LIB, gen=GEN, obj=OBJ):
+    """Generate and compile; force=True recompiles every object.  With
+    `variants` (mechanism -> knob list) builds only those kernels, without
+    SMVGEAR, into `lib` (the autotuner's candidate library)."""
+    os.makedirs(obj, exist_ok=True)
+    gen_srcs = generate(names, variants=variants, gen=gen, obj=obj, gear=variants is None)
     jobs = []
     for src, extra in gen_srcs:
         base = os.path.splitext(os.path.basename(src))[0]
-        jobs.append((src, os.path.join(OBJ, base + ".o"), src.endswith("registry.cpp"), extra, force))
-    jobs.append((os.path.join(CSRC, "cmaq_ros3.cpp"), os.path.join(OBJ, "cmaq_ros3.o"), False, (), force))
+        jobs.append((src, os.path.join(obj, base + ".o"), src.endswith("registry.cpp"), extra, force))
+    jobs.append((os.path.join(CSRC, "cmaq_ros3.cpp"), os.path.join(obj, "cmaq_ros3.o"), False, (), force))
     # largest sources first, so that the longest compiles overlap the others
     jobs.sort(key=lambda j: -os.path.getsize(j[0]))
     workers = int(os.environ.get("MAX_JOBS", "8"))
     with cf.ThreadPoolExecutor(max_workers=max(1, min(workers, len(jobs)))) as ex:
         objs = list(ex.map(lambda j: _compile(*j), jobs))
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", lib + ".tmp"] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(LIB + ".tmp", LIB)
+    os.replace(lib + ".tmp", lib)
+    if variants:
+        if verbose:
+            print(f"built {lib}")
+        return lib
     cmd = [HIPCC] + HIP_FLAGS + ["-o", PROBE + ".tmp", os.path.join(CSRC, "hbm_rows.hip")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -194,6 +227,12 @@ def build(names=None, verbose=True, force=False):
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+# the autotuner's candidate library (cmaq_amd/autotune.py): its own sources,
+# objects and .so, loaded with CMAQ_ROS3_LIB
+TUNE_GEN = os.path.join(CSRC, "gen_tune")
+TUNE_LIB = os.path.join(HERE, "libcmaq_ros3_tune.so")
 
 
 SAN_DIR = os.path.join(HERE, "_build", "san")

@@ -81,7 +81,7 @@ def structure_hash(m):
 TLM_MAX = 8   # tangent-linear directions carried by one call (kTlmMaxDirections)
 
 
-def _function(name, params, items, off, budget, dynamic=None, sched=0):
+def _function(name, params, items, off, budget, dynamic=None, sched=0, load_wrap=None):
     """HIP source of one generated function from allocator items.
 
     sched > 0 closes the compiler's instruction-scheduling region after every
@@ -107,7 +107,12 @@ def _function(name, params, items, off, budget, dynamic=None, sched=0):
     for k, it in enumerate(items):
         if sched and k and k % sched == 0:
             w.append("    __builtin_amdgcn_sched_barrier(0);")
-        if it[0] == "load":
+        if it[0] == "load" and load_wrap and it[2][0] in load_wrap:
+            # a read-only derived array: its slots are loaded from the
+            # source array's rows through a macro (e.g. CEFF of CNEW)
+            src, macro = load_wrap[it[2][0]]
+            w.append(f"    r{it[1]} = {macro}(LD({off[src] + it[2][1]}));")
+        elif it[0] == "load":
             w.append(f"    r{it[1]} = LD({addr(it[2])});")
         elif it[0] == "store":
             w.append(f"    ST({addr(it[2])}, r{it[1]});")
@@ -147,7 +152,7 @@ def waves_per_eu(budget):
 
 
 def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, refill_min=0,
-         jac_order=None):
+         jac_order=None, gear_nws=0):
     budget = budget or REG_BUDGET
     lookahead = lookahead or LOOKAHEAD
     jac_order = jac_order or JAC_ORDER
@@ -302,6 +307,7 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w(f"// register budget {budget} doubles, load lookahead {lookahead} operations, LU schedule {jac_order};")
     w("// per call (operations, loads, stores): " + ", ".join(f"{k} {v}" for k, v in stats.items()))
     w("#include \"../ros3_device.h\"")
+    w("#include \"../ros3_gear.h\"")
     w(f"namespace ros3 {{ namespace {cname} {{")
     w(f"constexpr int NS = {ns}, NR = {nr}, NNZ = {nnz}, NWS = {nws};")
     w(f"constexpr int TLM_MAX = {TLM_MAX}, TLM_RS = NS + NR;   // tangent-linear directions per call")
@@ -428,6 +434,11 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
         w("#define ROS3_MERGED_STEP 1")
     w("#include \"../ros3_solver_kernel.inc\"")
     w("__global__ __launch_bounds__(256, WAVES_PER_EU) void reverse_kernel(SolveParams P);   // unit 2")
+    if gear_nws:
+        w(f"}} namespace {cname}_gear {{")
+        w("__global__ __launch_bounds__(256, " + str(waves_per_eu(budget)) + ") void gear_kernel(GearParams P);   // SMVGEAR unit")
+        w("namespace tail { __global__ __launch_bounds__(kWave) void gear_tail_kernel(GearParams P); }")
+        w(f"}} namespace {cname} {{")
     w("#undef LD")
     w("#undef ST")
     w("#undef ROS3_PROLOGUE")
@@ -439,8 +450,14 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     # the host pass takes its address
     w("#ifdef __HIP_DEVICE_COMPILE__")
     w("#define ROS3_REVERSE_KERNEL nullptr")
+    w("#define ROS3_GEAR_KERNEL nullptr")
+    w("#define ROS3_GEAR_TAIL_KERNEL nullptr")
     w("#else")
     w(f"#define ROS3_REVERSE_KERNEL (const void *)&ros3::{cname}::reverse_kernel")
+    w(f"#define ROS3_GEAR_KERNEL (const void *)&ros3::{cname}_gear::gear_kernel" if gear_nws else
+      "#define ROS3_GEAR_KERNEL nullptr")
+    w(f"#define ROS3_GEAR_TAIL_KERNEL (const void *)&ros3::{cname}_gear::tail::gear_tail_kernel" if gear_nws else
+      "#define ROS3_GEAR_TAIL_KERNEL nullptr")
     w("#endif")
     w(f"extern \"C\" const ros3::MechKernels ros3_kernels_{cname} = {{")
     w(f"    \"{m['mechname']}\", ros3::{cname}::STRUCT_HASH, ros3::{cname}::NS, ros3::{cname}::NR,")
@@ -450,6 +467,127 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w(f"    (const void *)&ros3::{cname}::solve_kernel_t<true>, ROS3_REVERSE_KERNEL,")
     w(f"    ros3::{cname}::TAPE_RS, (const void *)&ros3::{cname}::solve_kernel_t<false, true>,")
     w(f"    ros3::{cname}::OFF_LAM,   // rows of the workspace the forward kernel touches")
-    w(f"    (const void *)&ros3::{cname}::solve_kernel_t<false, false, true>}};")
+    w(f"    (const void *)&ros3::{cname}::solve_kernel_t<false, false, true>,")
+    w(f"    ROS3_GEAR_KERNEL, {gear_nws}, ROS3_GEAR_TAIL_KERNEL}};")
     w("#undef ROS3_REVERSE_KERNEL")
+    w("#undef ROS3_GEAR_KERNEL")
+    w("#undef ROS3_GEAR_TAIL_KERNEL")
     return ("\n".join(out) + "\n", "\n".join(adj) + "\n"), sym
+
+
+def emit_gear(m, cname, budget=None, lookahead=None, jac_order=None):
+    """The SMVGEAR translation unit of one mechanism (namespace
+    ros3::<cname>_gear): SUBFUN, PDERIV + DECOMP and BACKSUB as straight-line
+    functions over the same day pattern and LU schedule as the ROS3 kernels,
+    and the persistent Gear kernel (ros3_gear_kernel.inc) that calls them."""
+    budget = budget or REG_BUDGET
+    lookahead = lookahead or LOOKAHEAD
+    base_order = (jac_order or JAC_ORDER).split("+")[0]
+    sym = analyse(m)
+    ns, nr, nnz = sym.ns, sym.nr, sym.day.nnz
+    from .programs import gear_pderiv_lu_ops, gear_subfun_ops
+    from .regalloc import allocate as _alloc
+    off, pos = {}, 0
+    for name, size in (("RK", nr), ("RATE", nr), ("CN", ns), ("GL", ns), ("CO", 7 * ns), ("DTL", ns),
+                       ("CE", ns), ("CH", ns), ("LU", nnz), ("VD", ns), ("B0", ns)):
+        off[name] = pos
+        pos += size
+    nws = pos
+    piv = frozenset(("LU", sym.day.diag[j]) for j in range(ns))
+    entry_all = lambda slot: True
+    stats, body = {}, []
+    for fname, ops, outs, params, dead in (
+            ("gear_subfun", gear_subfun_ops(sym), {"GL"}, "", frozenset()),
+            ("gear_pderiv_lu", gear_pderiv_lu_ops(sym, base_order), {"LU", "VD"}, ", double r1delt", piv),
+            ("gear_backsub", msolve_ops(sym, "GL"), {"GL"}, "", frozenset())):
+        items, cnt = _alloc(ops, outs, budget, lookahead, entry_all, dead=dead)
+        if any(it[0] == "store" and it[2][0] == "CF" for it in items):
+            raise RuntimeError("CEFF slots are read-only")
+        stats[fname] = _stat(ops, cnt)
+        body += _function(fname, params, items, off, budget, load_wrap={"CF": ("CN", "CEFF")})
+        body.append("")
+    gname = cname + "_gear"
+    w = []
+    a = w.append
+    a(f"// Generated by cmaq_amd/codegen/emit_hip.py for {m['mechname']}; do not edit.")
+    a("// SMVGEAR kernel (gear_kernel) over the ROS3 kernels' species order and sparse pattern;")
+    a(f"// register budget {budget} doubles, load lookahead {lookahead} operations, LU schedule {base_order};")
+    a("// per call (operations, loads, stores): " + ", ".join(f"{k} {v}" for k, v in stats.items()))
+    a("#include \"../ros3_device.h\"")
+    a("#include \"../ros3_gear.h\"")
+    a(f"namespace ros3 {{ namespace {gname} {{")
+    a(f"constexpr int NS = {ns}, NR = {nr}, NNZ = {nnz}, NWS = {nws};")
+    a(f"constexpr int WAVES_PER_EU = {waves_per_eu(budget)};")
+    a("static_assert(kGearCols == 7, \"CONC columns\");")
+    for k, v in off.items():
+        a(f"constexpr int GOFF_{k} = {v};")
+    a("#define ROS3_PROLOGUE \\")
+    a("    double *base = (double *)((((unsigned long long)(unsigned int)__builtin_amdgcn_readfirstlane( \\")
+    a("                                  (int)((unsigned long long)base_ >> 32))) << 32) | \\")
+    a("                              (unsigned long long)(unsigned int)__builtin_amdgcn_readfirstlane( \\")
+    a("                                  (int)(unsigned long long)base_)); \\")
+    a("    const __amdgpu_buffer_rsrc_t rs = \\")
+    a("        __builtin_amdgcn_make_buffer_rsrc(base, (short)0, NWS * kWave * 8, 0x00020000); \\")
+    a("    const int vo = __lane_id() * 8;")
+    a("#define LD(k) __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, (k) * (kWave * 8), 0))")
+    a("#define ST(k, v) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (double)(v)), rs, vo, (k) * (kWave * 8), 0)")
+    a("__device__ __forceinline__ double ceff(double v) { return v <= kGearZbound ? 0.0 : v; }   // grpderiv.F:121-129")
+    a("#define CEFF(v) ceff(v)")
+    a("#define ZBOUND kGearZbound")
+    a("__device__ __forceinline__ double opaque(double v) { asm volatile(\"\" : \"+v\"(v)); return v; }")
+    a("")
+    w += body
+    a("__device__ __noinline__ void load_gear_cell(const GearParams &P, int64_t cell, double *base_)")
+    a("{")
+    a("    ROS3_PROLOGUE")
+    a("    const int64_t n = P.stride;")
+    a("    for (int r = 0; r < NR; ++r) ST(GOFF_RK + r, P.rki[r * n + cell]);")
+    for i in range(ns):
+        a(f"    {{ const double v = P.y[{sym.inew2old[i]} * n + cell]; const double c = v > kGearConcmin ? v : kGearConcmin;"
+          f" ST(GOFF_CN + {i}, c); ST(GOFF_B0 + {i}, c); }}")
+    a("}")
+    a("__device__ __noinline__ void store_gear_cell(const GearParams &P, int64_t cell, double *base_)")
+    a("{")
+    a("    ROS3_PROLOGUE")
+    a("    const int64_t n = P.stride;")
+    for i in range(ns):
+        a(f"    P.y[{sym.inew2old[i]} * n + cell] = LD(GOFF_CN + {i});")
+    a("}")
+    a("#include \"../ros3_gear_kernel.inc\"")
+    for u in ("LD", "ST", "ROS3_PROLOGUE"):
+        a(f"#undef {u}")
+    # the same functions over one cell's workspace in LDS, for gear_tail_kernel
+    a("namespace tail {")
+    a("__shared__ double gear_lds[NWS];")
+    a("static_assert(NWS * 8 <= 160 * 1024, \"one cell's workspace fits the LDS of a CU\");")
+    a("#define GEAR_TAIL")
+    a("#define ROS3_PROLOGUE (void)base_;")
+    # Every access takes the LDS base through an empty asm: the compiler then
+    # keeps the allocator's loads and stores as they are (with constant
+    # addresses it would forward stored values into registers the allocator
+    # had freed, and spill)
+    a("typedef __attribute__((address_space(3))) double lds_double;")
+    a("__device__ __forceinline__ lds_double *lds_base()")
+    a("{")
+    a("    lds_double *p = (lds_double *)gear_lds;")
+    a("    asm volatile(\"\" : \"+s\"(p));")
+    a("    return p;")
+    a("}")
+    a("#define LD(k) (lds_base()[(k)])")
+    a("#define ST(k, v) (lds_base()[(k)] = (double)(v))")
+    w += body
+    a("__device__ __noinline__ void store_gear_cell(const GearParams &P, int64_t cell, double *base_)")
+    a("{")
+    a("    ROS3_PROLOGUE")
+    a("    const int64_t n = P.stride;")
+    for i in range(ns):
+        a(f"    P.y[{sym.inew2old[i]} * n + cell] = LD(GOFF_CN + {i});")
+    a("}")
+    a("#include \"../ros3_gear_kernel.inc\"")
+    for u in ("LD", "ST", "ROS3_PROLOGUE", "GEAR_TAIL"):
+        a(f"#undef {u}")
+    a("}  // namespace tail")
+    for u in ("CEFF", "ZBOUND"):
+        a(f"#undef {u}")
+    a(f"}} }} // namespace ros3::{gname}")
+    return "\n".join(w) + "\n", nws

@@ -124,6 +124,50 @@ def _entry_ops(sym, terms, s, diag):
     return ops
 
 
+def _gear_entry_ops(sym, terms, s, diag):
+    """One entry of SMVGEAR's predictor matrix P = I + R1DELT * J
+    (grpderiv.F:88-224): off-diagonal entries start at 0 and diagonal ones at
+    1; each partial derivative EXPLIC is multiplied by R1DELT (or by
+    FRACN = SC * R1DELT) as it is added, with the concentrations at or below
+    ZBOUND taken as zero (CEFF)."""
+    ops = [Op("{0} = 1.0;" if s == diag else "{0} = 0.0;", [("LU", s)], False)]
+    for sign, c, r, ial in terms[s]:
+        srcs, ex = _explic(sym, r, ial)
+        # CEFF(i) = CNEW(i), or 0 at or below ZBOUND: slots of array "CF"
+        srcs = [("CF", a[1]) if a[0] == "Y" else a for a in srcs]
+        ex = ex % tuple(range(1, len(srcs) + 1))
+        if sign == "-":
+            fmt = "{0} = {0} - r1delt * " + ex + ";"
+        elif c is None:
+            fmt = "{0} = {0} + r1delt * " + ex + ";"
+        else:
+            # FRACN = SC * R1DELT, formed per term (opaque() keeps the compiler
+            # from holding every distinct FRACN live across the function)
+            fmt = "{0} = {0} + (" + _lit(c) + " * opaque(r1delt)) * " + ex + ";"
+        ops.append(Op(fmt, [("LU", s)] + srcs, True))
+    return ops
+
+
+def gear_pderiv_lu_ops(sym, order="column"):
+    """PDERIV then DECOMP (grpderiv.F, grdecomp.F) in the given schedule;
+    grdecomp.F applies RBDECOMP's operations in the same order."""
+    return _jac_lu_ops(sym, order, entry=_gear_entry_ops)
+
+
+def gear_subfun_ops(sym):
+    """SUBFUN (grsubfun.F:83-171) without emissions: RBFEVAL's accumulation
+    on CNEW into GLOSS, then a species at or below ZBOUND loses nothing."""
+    ops = _remap_arrays(feval_ops(sym, "scatter"), {"YS": "CN", "F": "GL"})
+    for i in range(sym.ns):
+        ops.append(Op("{0} = ({1} <= ZBOUND && {0} < 0.0) ? 0.0 : {0};", [("GL", i), ("CN", i)], True))
+    return ops
+
+
+def _remap_arrays(ops, names):
+    return [Op(op.fmt, [(names.get(a[0], a[0]), a[1]) if isinstance(a, tuple) else a for a in op.args], op.rmw)
+            for op in ops]
+
+
 def _fuse_k1_forward(sym, ops):
     """Interleaves stage 1's forward substitution (rbsolve.F:78-128 with
     K1 = -F, rbsolver.F:386-388) with the factorization: right after L(i,j)
@@ -163,7 +207,8 @@ def jac_lu_ops(sym, order="column", fuse_k1=False):
     return _fuse_k1_forward(sym, ops) if fuse_k1 else ops
 
 
-def _jac_lu_ops(sym, order="column"):
+def _jac_lu_ops(sym, order="column", entry=None):
+    entry = entry or _entry_ops
     day = sym.day
     n = sym.ns
     terms = _jac_terms(sym)
@@ -174,7 +219,7 @@ def _jac_lu_ops(sym, order="column"):
             col_slots[day.cols[s]].append(s)
         for j in range(n):
             for s in col_slots[j]:
-                ops += _entry_ops(sym, terms, s, day.diag[j])
+                ops += entry(sym, terms, s, day.diag[j])
             beg, end = day.lu_col_ops[j]
             for (ij, ik, kj) in day.lu_ops[beg:end]:
                 ops.append(Op("{0} = {0} - {1} * {2};", [("LU", ij), ("LU", ik), ("LU", kj)], True))
@@ -216,7 +261,7 @@ def _jac_lu_ops(sym, order="column"):
             ents = sorted(tiles[key], key=lambda s: (day.rows[s], day.cols[s]))
             in_tile = set(ents)
             for s in ents:
-                ops += _entry_ops(sym, terms, s, day.diag[day.cols[s]])
+                ops += entry(sym, terms, s, day.diag[day.cols[s]])
             cols = sorted({day.cols[s] for s in ents})
             by_k = {}
             for s in ents:
@@ -253,7 +298,7 @@ def _jac_lu_ops(sym, order="column"):
             grp = list(range(j0, min(n, j0 + g)))
             for j in grp:
                 for s in col_slots[j]:
-                    ops += _entry_ops(sym, terms, s, day.diag[j])
+                    ops += entry(sym, terms, s, day.diag[j])
             by_k = {}
             for j in grp:
                 beg, end = day.lu_col_ops[j]
@@ -284,7 +329,7 @@ def _jac_lu_ops(sym, order="column"):
         row_slots[day.rows[s]].append(s)
     for i in range(n):
         for s in row_slots[i]:
-            ops += _entry_ops(sym, terms, s, day.diag[i])
+            ops += entry(sym, terms, s, day.diag[i])
         lks = sorted(day.cols[s] for s in row_slots[i] if day.cols[s] < i)
         for k in lks:
             s_ik = day.slot[(i, k)]

@@ -21,6 +21,7 @@
 
 #include "../../include/cmaq_ros3.h"
 #include "ros3_device.h"
+#include "ros3_gear.h"
 #include "ros3_chem_io.h"
 #include "ros3_ddm.h"
 
@@ -67,6 +68,17 @@ struct cmaq_ros3_ctx {
     int grid_replay = 0, grid_reverse = 0;   // adjoint kernels' workgroups
     int grid_tlm = 0;                          // tangent-linear kernel's workgroups
     int grid_yave = 0;                         // YAVE-accumulating forward's workgroups
+    const MechKernels *gk = nullptr;           // the record carrying the SMVGEAR kernel
+    int grid_gear = 0;                         // SMVGEAR kernel's workgroups
+    int grid_gear_tail = 0;                    // gear_tail_kernel's one-wavefront workgroups
+    int32_t gear_tail_after = 2048;            // attempted steps before a cell moves to the tail kernel
+    int32_t gear_tail_cells = 0;               // most cells handed over per call; 0 = automatic
+    int64_t gear_tail_last = 0;                // cells handed over in the last Gear call
+    void *d_gear_tail = nullptr;               // GearLaneState [cap], then workspaces [cap][gear_nws]
+    size_t gear_tail_bytes = 0;
+    cmaq_gear_tol gtol = {1.0e-3f, 1.0e-7f, 1.0e-9f, 1.0e-12f};   // RUNTIME_VARS.F:173-176
+    uint8_t *d_sunlit = nullptr;               // LSUNLIGHT per cell (SMVGEAR)
+    int64_t sunlit_cells = 0;
     int num_cus = 0;
     int64_t waves = 0;
     // device buffers
@@ -293,6 +305,10 @@ int decode_rates(const Tab &t, std::vector<RateOp> &ops, std::vector<RateMul> &m
             }
             break;
         }
+        case 7:   // A*(1 + 0.6*PRESS) or ARRHENUIS_T04(.., A, 0, B)*PRESS, constants as 1PD12.4
+                  // (wrt_reactions_module.f:731-737, formats 5007 / 5114)
+            op.kind = RK_T07; op.p[0] = d12_4(a); op.p[1] = d12_4(b);
+            break;
         case 8:
             if (f < 0) { why = "no RFDAT row for " + std::to_string(r + 1); return -1; }
             op.kind = RK_T08; op.conv = conv_of(ord);
@@ -406,13 +422,23 @@ static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
     if (int rc = grid_for(c, mk->replay_kernel, &grid_a)) return rc;
     if (int rc = grid_for(c, mk->reverse_kernel, &grid_b)) return rc;
     if (int rc = grid_for(c, mk->tlm_kernel, &grid_t)) return rc;
+    int grid_g = 0, grid_gt = 0;
+    if (c->gk && c->gk->gear_kernel)
+        if (int rc = grid_for(c, c->gk->gear_kernel, &grid_g)) return rc;
+    if (c->gk && c->gk->gear_tail_kernel) {
+        int per_cu = 0;
+        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c->gk->gear_tail_kernel, kWave, 0));
+        grid_gt = c->num_cus * std::max(per_cu, 1);
+    }
     const int wpb = kSolveBlock / kWave;
-    const int64_t waves = (int64_t)std::max(std::max(std::max(grid, grid_t), std::max(grid_a, grid_b)), grid_y) * wpb;
+    const int64_t waves =
+        (int64_t)std::max(std::max(std::max(grid, grid_t), std::max(grid_a, grid_b)), std::max(grid_y, grid_g)) * wpb;
     // per-wave workspace stride: NWS rows of 64 doubles plus an optional pad
     // of rows that staggers the waves' bases across memory channels
     int64_t pad = 0;
     if (const char *e = std::getenv("CMAQ_ROS3_WS_PAD")) pad = std::max(0ll, std::atoll(e));
-    const int64_t stride = ((int64_t)mk->nws + pad) * kWave;
+    const int64_t rows = std::max<int64_t>(mk->nws, c->gk ? c->gk->gear_nws : 0);
+    const int64_t stride = (rows + pad) * kWave;
     const size_t need = (size_t)waves * stride;
     if (need > c->ws_doubles) {
         hipFree(c->d_ws);
@@ -428,6 +454,8 @@ static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
     c->grid_reverse = grid_b;
     c->grid_tlm = grid_t;
     c->grid_yave = grid_y;
+    c->grid_gear = grid_g;
+    c->grid_gear_tail = grid_gt;
     c->ws_stride = stride;
     c->waves = (int64_t)(c->ws_doubles / (size_t)stride);
     return 0;
@@ -465,6 +493,8 @@ extern "C" int cmaq_ros3_init(cmaq_ros3_ctx **out, const cmaq_ros3_mech *mech, c
     cmaq_ros3_ctx *c = new cmaq_ros3_ctx();
     c->mk = mk;
     c->variants = found;
+    for (const MechKernels *k : found)
+        if (!c->gk && k->gear_kernel) c->gk = k;
     c->ns = mech->numb_mech_spc;
     c->nr = mech->nrxns;
     c->nphotab = mech->nphotab;
@@ -581,12 +611,13 @@ extern "C" void cmaq_ros3_free(cmaq_ros3_ctx *c)
     if (!c) return;
     hipSetDevice(c->device);
     hipFree(c->d_ops); hipFree(c->d_mul); hipFree(c->d_rtol); hipFree(c->d_atol);
+    hipFree(c->d_gear_tail);
     hipFree(c->d_ws); hipFree(c->d_small); hipFree(c->d_fail_detail); hipFree(c->d_rki); hipFree(c->d_io);
     hipFree(c->d_tape); hipFree(c->d_tape_off); hipFree(c->d_order); hipFree(c->d_sort);
     hipFree(c->d_ddm_rx); hipFree(c->d_ddm_act); hipFree(c->d_ddm_pid); hipFree(c->d_ddm_inew2old);
     hipFree(c->d_ddm_psc); hipFree(c->d_yave); hipFree(c->d_ddm_a);
     hipFree(c->d_step_na); hipFree(c->d_step_nr);
-    hipFree(c->d_map); hipFree(c->d_block); hipFree(c->d_grid);
+    hipFree(c->d_map); hipFree(c->d_block); hipFree(c->d_grid); hipFree(c->d_sunlit);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
     if (c->gstream) hipStreamDestroy(c->gstream);
@@ -615,6 +646,7 @@ extern "C" int cmaq_ros3_query(const cmaq_ros3_ctx *c, const char *key, int64_t 
     else if (!std::strcmp(key, "solve_waves")) *value = (int64_t)c->grid_blocks * (kSolveBlock / kWave);
     else if (!std::strcmp(key, "workspace_doubles_per_lane")) *value = k->nws;
     else if (!std::strcmp(key, "tape_record_doubles")) *value = k->tape_rs;
+    else if (!std::strcmp(key, "gear_tail_cells_last")) *value = c->gear_tail_last;
     else return CMAQ_ROS3_E_ARG;
     return 0;
 }
@@ -696,14 +728,14 @@ extern "C" int cmaq_ros3_get_fail_detail(const cmaq_ros3_ctx *c, cmaq_ros3_fail_
 
 static int launch_rates(cmaq_ros3_ctx *c, int64_t n, const double *temp, const double *pres,
                         const double *h2o, const double *rj, const double *het, const double *sea,
-                        double *rki, hipStream_t s)
+                        double *rki, hipStream_t s, uint8_t *sunlit = nullptr)
 {
     RatesParams P;
     P.ncells = n; P.nr = c->nr; P.nphotab = c->nphotab; P.nhetero = c->nhetero;
     P.nmul = (int32_t)c->mul.size();
     P.ops = c->d_ops; P.mul = c->d_mul;
     P.temp = temp; P.pres = pres; P.h2o = h2o; P.rj = rj; P.het = het; P.sea = sea;
-    P.rki = rki; P.sunlit = nullptr;
+    P.rki = rki; P.sunlit = sunlit;
     int64_t blocks = (n + 255) / 256;
     blocks = std::min<int64_t>(blocks, 65536);
     if (c->nr <= kMaxRates)
@@ -1657,4 +1689,214 @@ extern "C" int cmaq_ros3_chem(cmaq_ros3_ctx *c, const cmaq_ros3_grid *g, double 
     if (nacc) HIP_TRY(c, hipMemcpy(nacc, b[9], (size_t)n * 4, hipMemcpyDeviceToHost));
     if (nrej) HIP_TRY(c, hipMemcpy(nrej, b[10], (size_t)n * 4, hipMemcpyDeviceToHost));
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// SMVGEAR (grdriver.F:400-560 + grsmvgear.F) on the same context
+// ---------------------------------------------------------------------------
+extern "C" int cmaq_gear_set_tolerances(cmaq_ros3_ctx *c, const cmaq_gear_tol *tol)
+{
+    if (!c) return CMAQ_ROS3_E_ARG;
+    const cmaq_gear_tol dflt = {1.0e-3f, 1.0e-7f, 1.0e-9f, 1.0e-12f};
+    const cmaq_gear_tol t = tol ? *tol : dflt;
+    if (!(t.rtol > 0.0f) || !(t.max_atol > t.min_atol) || !(t.min_atol > t.conc_floor) || !(t.conc_floor > 0.0f))
+        return set_err(c, CMAQ_ROS3_E_ARG, "Gear tolerances need rtol > 0 and max_atol > min_atol > conc_floor > 0");
+    c->gtol = t;
+    return 0;
+}
+
+extern "C" int cmaq_gear_set_tail(cmaq_ros3_ctx *c, int32_t after_attempts, int32_t max_cells)
+{
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (after_attempts < 0 || max_cells < 0)
+        return set_err(c, CMAQ_ROS3_E_ARG, "cmaq_gear_set_tail needs after_attempts >= 0 and max_cells >= 0");
+    c->gear_tail_after = after_attempts;
+    c->gear_tail_cells = max_cells;
+    return 0;
+}
+
+// Gear kernel over n cells with rki (and sunlit) already on the device.
+static int launch_gear(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki, const uint8_t *sunlit,
+                       double *y, int32_t *nsteps, int32_t *nfail, cmaq_ros3_fail *fail, hipStream_t s)
+{
+    HIP_TRY(c, hipEventRecord(c->ev[1], s));
+    GearParams P;
+    std::memset(&P, 0, sizeof P);
+    P.ncells = n; P.stride = n; P.chemstep = chemstep; P.rki = rki; P.sunlit = sunlit; P.y = y;
+    P.nsteps = nsteps; P.nfail = nfail; P.ws = c->d_ws; P.ws_stride = c->ws_stride; P.queue = c->d_queue;
+    P.fail_flag = c->d_fail_flag; P.fail_cell = c->d_fail_cell; P.fail_info = c->d_fail_info;
+    P.fail_code = c->d_fail_code;
+    P.eps = (double)c->gtol.rtol;
+    // OPTIMAL_ATOL_PPM's levels (GRVARS.F:595-608), evaluated as the reference does
+    const double abhi = std::log10((double)c->gtol.max_atol), ablo = std::log10((double)c->gtol.min_atol);
+    P.atol_level[0] = 0.0;
+    P.atol_level[1] = std::pow(10.0, abhi);
+    P.atol_level[6] = std::pow(10.0, ablo);
+    for (int i = 2; i <= 5; ++i) P.atol_level[i] = std::pow(10.0, ablo + (abhi - ablo) * (double)(5 - i + 1) / 5.0);
+    P.conc_floor = (double)c->gtol.conc_floor;
+    P.max_steps = c->max_attempts;
+    P.refill_min = c->refill_min > 0 ? c->refill_min : 1;
+    // long cells continue in gear_tail_kernel (the counters sit in d_small,
+    // which the caller has cleared)
+    const bool tail = c->gk->gear_tail_kernel && c->gear_tail_after > 0 && c->grid_gear_tail > 0;
+    if (tail) {
+        const int64_t cap = c->gear_tail_cells > 0 ? std::min<int64_t>(n, c->gear_tail_cells)
+                                                   : std::min<int64_t>(n, std::max<int64_t>(1024, n / 32));
+        const size_t state_bytes = ((size_t)cap * sizeof(GearLaneState) + 255) & ~(size_t)255;
+        const size_t need = state_bytes + (size_t)cap * (size_t)c->gk->gear_nws * sizeof(double);
+        if (need > c->gear_tail_bytes) {
+            HIP_TRY(c, hipStreamSynchronize(s));
+            hipFree(c->d_gear_tail);
+            c->d_gear_tail = nullptr;
+            c->gear_tail_bytes = 0;
+            if (hipMalloc(&c->d_gear_tail, need) != hipSuccess)
+                return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of the SMVGEAR long-cell buffers failed");
+            c->gear_tail_bytes = need;
+        }
+        P.tail_after = c->gear_tail_after;
+        P.tail_cap = (int32_t)cap;
+        P.tail_state = (GearLaneState *)c->d_gear_tail;
+        P.tail_ws = (double *)((char *)c->d_gear_tail + state_bytes);
+        P.tail_count = (unsigned int *)((char *)c->d_small + 48);
+        P.tail_queue = (unsigned int *)((char *)c->d_small + 52);
+    }
+    void *args[] = {&P};
+    HIP_TRY(c, hipLaunchKernel(c->gk->gear_kernel, dim3(c->grid_gear), dim3(kSolveBlock), args, 0, s));
+    if (tail)
+        HIP_TRY(c, hipLaunchKernel(c->gk->gear_tail_kernel, dim3(c->grid_gear_tail), dim3(kWave), args, 0, s));
+    HIP_TRY(c, hipEventRecord(c->ev[2], s));
+    unsigned char small[64];
+    HIP_TRY(c, hipMemcpyAsync(small, c->d_small, 64, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    float ms_r = 0.f, ms_s = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms_r, c->ev[0], c->ev[1]));
+    HIP_TRY(c, hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
+    c->rates_ms = ms_r;
+    c->solve_ms = ms_s;
+    // the fail record: DELT, XELAPS, YLOWA in place of MAXERR, DT, TNOW
+    int32_t flag, code;
+    int64_t cell;
+    double info[3];
+    std::memcpy(&cell, small + 8, 8);
+    std::memcpy(info, small + 16, 24);
+    std::memcpy(&flag, small + 40, 4);
+    std::memcpy(&code, small + 44, 4);
+    uint32_t handed = 0;
+    std::memcpy(&handed, small + 48, 4);
+    c->gear_tail_last = tail ? std::min<int64_t>(handed, P.tail_cap) : 0;
+    if (!flag) return 0;
+    if (fail) { fail->code = code; fail->cell = cell; fail->maxerr = info[2]; fail->dt = info[0]; fail->tnow = info[1]; }
+    char buf[200];
+    std::snprintf(buf, sizeof buf, "SMVGEAR %s in cell %lld (DELT %.4e min, XELAPS %.4e min, YLOWA %.4e ppm)",
+                  code == CMAQ_ROS3_E_STEPS ? "step limit" : "failure (YLOWA lowered 20 times or 100 restarts)",
+                  (long long)cell, info[0], info[1], info[2]);
+    return set_err(c, code, buf);
+}
+
+static int gear_ready(cmaq_ros3_ctx *c)
+{
+    if (!c->gk || !c->gk->gear_kernel)
+        return set_err(c, CMAQ_ROS3_E_MECH, "no SMVGEAR kernel built for this mechanism");
+    return 0;
+}
+
+extern "C" int cmaq_gear_solve_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *rki,
+                                      const uint8_t *sunlit, double *y, int32_t *nsteps, int32_t *nfail,
+                                      cmaq_ros3_fail *fail, void *stream)
+{
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (int rc = gear_ready(c)) return rc;
+    if (n > 0 && (!rki || !y)) return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
+    HIP_TRY(c, hipEventRecord(c->ev[0], s));
+    return launch_gear(c, n, chemstep, rki, sunlit, y, nsteps, nfail, fail, s);
+}
+
+extern "C" int cmaq_gear_run_device(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *temp,
+                                    const double *pres, const double *h2o, const double *rj, const double *het,
+                                    const double *sea, double *y, int32_t *nsteps, int32_t *nfail,
+                                    cmaq_ros3_fail *fail, void *stream)
+{
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (int rc = gear_ready(c)) return rc;
+    if (n > 0 && (!temp || !pres || !h2o || !y || (c->nphotab > 0 && !rj)))
+        return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (c->rki_cells < n) {
+        hipFree(c->d_rki);
+        c->d_rki = nullptr;
+        c->rki_cells = 0;
+        if (hipMalloc((void **)&c->d_rki, (size_t)n * c->nr * sizeof(double)) != hipSuccess)
+            return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of rate constants failed");
+        c->rki_cells = n;
+    }
+    if (c->sunlit_cells < n) {
+        hipFree(c->d_sunlit);
+        c->d_sunlit = nullptr;
+        c->sunlit_cells = 0;
+        if (hipMalloc((void **)&c->d_sunlit, (size_t)n) != hipSuccess)
+            return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of the sunlit flags failed");
+        c->sunlit_cells = n;
+    }
+    HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
+    HIP_TRY(c, hipEventRecord(c->ev[0], s));
+    int rc = launch_rates(c, n, temp, pres, h2o, rj, het, sea, c->d_rki, s, c->d_sunlit);
+    if (rc) return rc;
+    return launch_gear(c, n, chemstep, c->d_rki, c->d_sunlit, y, nsteps, nfail, fail, s);
+}
+
+extern "C" int cmaq_gear_run(cmaq_ros3_ctx *c, int64_t n, double chemstep, const double *temp,
+                             const double *pres, const double *h2o, const double *rj, const double *het,
+                             const double *sea, double *y, int32_t *nsteps, int32_t *nfail, cmaq_ros3_fail *fail)
+{
+    if (fail) { fail->code = 0; fail->pad = 0; fail->cell = -1; fail->maxerr = fail->dt = fail->tnow = 0.0; }
+    if (!c) return CMAQ_ROS3_E_ARG;
+    if (int rc = gear_ready(c)) return rc;
+    if (n < 0 || !(chemstep > 0.0) || (n > 0 && (!temp || !pres || !h2o || !y || (c->nphotab > 0 && !rj))))
+        return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t nd = (size_t)n * sizeof(double);
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t sizes[9] = {nd, nd, nd, nd * c->nphotab, het ? nd * c->nhetero : 0, sea ? nd : 0,
+                             nd * c->ns, (size_t)n * 4, (size_t)n * 4};
+    size_t need = 0;
+    for (size_t b : sizes) need += al(b);
+    if (c->io_bytes < need) {
+        hipFree(c->d_io);
+        c->d_io = nullptr;
+        c->io_bytes = 0;
+        if (hipMalloc(&c->d_io, need) != hipSuccess) return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of I/O buffers failed");
+        c->io_bytes = need;
+    }
+    char *bufs[9];
+    char *p = (char *)c->d_io;
+    for (int i = 0; i < 9; ++i) { bufs[i] = sizes[i] ? p : nullptr; p += al(sizes[i]); }
+    double *d_t = (double *)bufs[0], *d_p = (double *)bufs[1], *d_h = (double *)bufs[2];
+    double *d_rj = (double *)bufs[3], *d_het = (double *)bufs[4], *d_sea = (double *)bufs[5];
+    double *d_y = (double *)bufs[6];
+    int32_t *d_ns = (int32_t *)bufs[7], *d_nf = (int32_t *)bufs[8];
+    HIP_TRY(c, hipMemcpy(d_t, temp, nd, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(d_p, pres, nd, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(d_h, h2o, nd, hipMemcpyHostToDevice));
+    if (c->nphotab) HIP_TRY(c, hipMemcpy(d_rj, rj, nd * c->nphotab, hipMemcpyHostToDevice));
+    if (het && c->nhetero) HIP_TRY(c, hipMemcpy(d_het, het, nd * c->nhetero, hipMemcpyHostToDevice));
+    if (sea) HIP_TRY(c, hipMemcpy(d_sea, sea, nd, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(d_y, y, nd * c->ns, hipMemcpyHostToDevice));
+    int rc = cmaq_gear_run_device(c, n, chemstep, d_t, d_p, d_h, d_rj, d_het, d_sea, d_y, d_ns, d_nf, fail, nullptr);
+    if (rc != 0 && rc != CMAQ_ROS3_E_CONVERGENCE && rc != CMAQ_ROS3_E_STEPS) return rc;
+    HIP_TRY(c, hipMemcpy(y, d_y, nd * c->ns, hipMemcpyDeviceToHost));
+    if (nsteps) HIP_TRY(c, hipMemcpy(nsteps, d_ns, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (nfail) HIP_TRY(c, hipMemcpy(nfail, d_nf, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return rc;
+}
+
+extern "C" int cmaq_gear_last_timing(const cmaq_ros3_ctx *c, double *rates_ms, double *solve_ms)
+{
+    return cmaq_ros3_last_timing(c, rates_ms, solve_ms);
 }

@@ -65,6 +65,7 @@ enum RateKind : int32_t {
     RK_T09,
     RK_T10,
     RK_T11,
+    RK_T07,           // a*(1 + 0.6 P), or a*(T/300)^b * P (P in atm)
 };
 enum ConvKind : int32_t {   // chemmech unit factor; CV_NONE writes none
     CV_NONE = -1, CV_INV_CFACT = 0, CV_SFACT, CV_CFACT, CV_CFACT_SQU, CV_RFACT, CV_RFACT_SQU
@@ -172,6 +173,10 @@ __device__ inline double rate_eval(const RateOp &op, const CellConv &u, double k
         v = k1 + k2 * u.cair + k3;
         break;
     }
+    case RK_T07:
+        // chemmech writes no unit factor for this type
+        if (p[1] != 0.0) return p[0] * exp(0.0 * u.inv_temp) * pow(u.t300, p[1]) * pres;
+        return p[0] * (1.0 + 0.6 * pres);
     default:
         skip = true;
         return 0.0;

@@ -22,6 +22,9 @@ struct MechKernels {
     const void *tlm_kernel;           // solve_kernel carrying a tangent-linear direction
     int nws_fwd;                      // workspace rows (entries) the forward kernel touches
     const void *yave_kernel;          // solve_kernel accumulating the time-averaged state
+    const void *gear_kernel;          // SMVGEAR kernel (default variants), or nullptr
+    int gear_nws;                     // workspace rows (entries) the SMVGEAR kernel uses
+    const void *gear_tail_kernel;     // SMVGEAR for long cells, one per wavefront in LDS, or nullptr
 };
 
 }  // namespace ros3

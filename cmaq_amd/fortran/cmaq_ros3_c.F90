@@ -71,6 +71,12 @@ module cmaq_ros3_c
       type(c_ptr)        :: cgrid, ta, dens, pres, qv, seawater_zone, seaice, rj, khetero
    end type cmaq_ros3_grid_t
 
+   ! cmaq_gear_tol: GEAR_RTOL, GEAR_MAX_ATOL, GEAR_MIN_ATOL, GEAR_CONC_FLOOR
+   ! (REAL(4), RUNTIME_VARS.F:173-176)
+   type, bind(c) :: cmaq_gear_tol_t
+      real(c_float) :: rtol = 1.0e-3, max_atol = 1.0e-7, min_atol = 1.0e-9, conc_floor = 1.0e-12
+   end type cmaq_gear_tol_t
+
    interface
       ! RBINIT + RBSPARSE (rbinit.F:26, rbsparse.F:26)
       integer(c_int) function cmaq_ros3_init(ctx, mech, rtol, atol, opts) &
@@ -96,6 +102,28 @@ module cmaq_ros3_c
          integer(c_int32_t), intent(out)    :: nacc(*), nrej(*)
          type(cmaq_ros3_fail_t), intent(out) :: fail
       end function cmaq_ros3_run
+
+      ! SMVGEAR's tolerances (GRVARS.F:550-707 reads them)
+      integer(c_int) function cmaq_gear_set_tolerances(ctx, tol) bind(c, name='cmaq_gear_set_tolerances')
+         import :: c_int, c_ptr, cmaq_gear_tol_t
+         type(c_ptr), value                 :: ctx
+         type(cmaq_gear_tol_t), intent(in)  :: tol
+      end function cmaq_gear_set_tolerances
+
+      ! CALC_RCONST + SMVGEAR over all cells (grdriver.F:400-560), host arrays
+      integer(c_int) function cmaq_gear_run(ctx, ncells, chemstep, temp, pres, h2o, rj, het, &
+            seawater, y, nsteps, nfail, fail) bind(c, name='cmaq_gear_run')
+         import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double, cmaq_ros3_fail_t
+         type(c_ptr), value                 :: ctx
+         integer(c_int64_t), value          :: ncells
+         real(c_double), value              :: chemstep            ! minutes
+         real(c_double), intent(in)         :: temp(*), pres(*), h2o(*)   ! K, atm, ppm
+         real(c_double), intent(in)         :: rj(*)               ! (NCELLS, NPHOTAB) 1/min
+         type(c_ptr), value                 :: het, seawater       ! C_LOC(...) or C_NULL_PTR (= 0)
+         real(c_double), intent(inout)      :: y(*)                ! (NCELLS, NUMB_MECH_SPC) ppm
+         integer(c_int32_t), intent(out)    :: nsteps(*), nfail(*)
+         type(cmaq_ros3_fail_t), intent(out) :: fail
+      end function cmaq_gear_run
 
       ! MAP_CHEMISTRY_SPECIES result (RXNS_FUNC_MODULE.F90:1117-1305)
       integer(c_int) function cmaq_ros3_set_cgrid_map(ctx, nspcsd, cgrid_index, convert_conc, &

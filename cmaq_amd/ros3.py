@@ -68,6 +68,10 @@ class _FailDetail(C.Structure):
                 ("h2o_ppm", C.c_double), ("seawater", C.c_double), ("dens", C.c_double)]
 
 
+class _GearTol(C.Structure):
+    _fields_ = [("rtol", C.c_float), ("max_atol", C.c_float), ("min_atol", C.c_float), ("conc_floor", C.c_float)]
+
+
 class _Grid(C.Structure):
     _fields_ = [("ncols", C.c_int32), ("nrows", C.c_int32), ("nlays", C.c_int32), ("ld_het", C.c_int32),
                 ("cgrid", C.c_void_p), ("ta", C.c_void_p), ("dens", C.c_void_p), ("pres", C.c_void_p),
@@ -129,6 +133,12 @@ def lib():
         L.cmaq_ros3_chem_device.argtypes = [vp, C.POINTER(_Grid), C.c_double, vp, vp, C.POINTER(_Fail), vp]
         L.cmaq_ros3_gather_device.argtypes = [vp, C.POINTER(_Grid)] + [vp] * 7 + [vp]
         L.cmaq_ros3_scatter_device.argtypes = [vp, C.POINTER(_Grid), vp, vp]
+        L.cmaq_gear_set_tolerances.argtypes = [vp, C.POINTER(_GearTol)]
+        L.cmaq_gear_set_tail.argtypes = [vp, C.c_int32, C.c_int32]
+        L.cmaq_gear_run.argtypes = [vp, C.c_int64, C.c_double] + [_f64p] * 7 + [_i32p, _i32p, C.POINTER(_Fail)]
+        L.cmaq_gear_run_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 9 + [C.POINTER(_Fail), vp]
+        L.cmaq_gear_solve_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 5 + [C.POINTER(_Fail), vp]
+        L.cmaq_gear_last_timing.argtypes = [vp, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -490,6 +500,87 @@ class Ros3Solver:
         if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
             raise self._err(rc, fail)
         return fail if rc != 0 else None
+
+    # ---- SMVGEAR (CCTM/src/gas/smvgear) on the same context ----
+    def set_gear_tolerances(self, rtol=1.0e-3, max_atol=1.0e-7, min_atol=1.0e-9, conc_floor=1.0e-12):
+        """GEAR_RTOL, GEAR_MAX_ATOL, GEAR_MIN_ATOL, GEAR_CONC_FLOOR (REAL(4),
+        RUNTIME_VARS.F:173-176)."""
+        t = _GearTol(rtol, max_atol, min_atol, conc_floor)
+        rc = lib().cmaq_gear_set_tolerances(self._ctx, C.byref(t))
+        if rc != 0:
+            raise self._err(rc)
+
+    def set_gear_tail(self, after_attempts=2048, max_cells=0):
+        """Cells still integrating after `after_attempts` attempted Gear steps
+        (0 = never) finish in the one-cell-per-wavefront LDS kernel, at most
+        `max_cells` per call (0 = automatic).  Results do not change."""
+        rc = lib().cmaq_gear_set_tail(self._ctx, int(after_attempts), int(max_cells))
+        if rc != 0:
+            raise self._err(rc)
+
+    def gear_run(self, chemstep, temp, pres_atm, h2o_ppm, rj, y, het=None, seawater=None, raise_on_fail=True):
+        """CALC_RCONST + SMVGEAR on host arrays; returns (y_new, n_steps,
+        n_fail); a failed cell has n_steps = -1."""
+        temp = np.ascontiguousarray(temp, np.float64)
+        n = temp.size
+        pres_atm = np.ascontiguousarray(pres_atm, np.float64)
+        h2o_ppm = np.ascontiguousarray(h2o_ppm, np.float64)
+        rj = np.ascontiguousarray(rj, np.float64).reshape(self.nphotab, n)
+        yy = np.array(y, np.float64, copy=True, order="C").reshape(self.ns, n)
+        het_p = None if het is None else np.ascontiguousarray(het, np.float64).reshape(self.nhetero, n)
+        sw_p = None if seawater is None else np.ascontiguousarray(seawater, np.float64)
+        ns_, nf = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        f = _Fail()
+        p = lambda a: None if a is None else a.ctypes.data_as(_f64p)
+        rc = lib().cmaq_gear_run(self._ctx, n, chemstep, p(temp), p(pres_atm), p(h2o_ppm), p(rj), p(het_p),
+                                 p(sw_p), p(yy), ns_.ctypes.data_as(_i32p), nf.ctypes.data_as(_i32p), C.byref(f))
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, {"cell": f.cell, "ylowa": f.maxerr, "dt": f.dt, "tnow": f.tnow})
+        return yy, ns_, nf
+
+    def gear_run_device(self, chemstep, temp, pres_atm, h2o_ppm, rj, y, nsteps=None, nfail=None, het=None,
+                        seawater=None, stream=None, raise_on_fail=True):
+        """CALC_RCONST + SMVGEAR on CUDA tensors; y updated in place."""
+        import torch
+        n = temp.numel()
+        self._check([("temp", temp, 1), ("pres", pres_atm, 1), ("h2o", h2o_ppm, 1), ("rj", rj, self.nphotab),
+                     ("y", y, self.ns), ("nacc", nsteps, 1), ("nrej", nfail, 1), ("het", het, self.nhetero),
+                     ("seawater", seawater, 1)], n)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_gear_run_device(self._ctx, n, chemstep, _ptr(temp), _ptr(pres_atm), _ptr(h2o_ppm),
+                                        _ptr(rj), _ptr(het), _ptr(seawater), _ptr(y), _ptr(nsteps), _ptr(nfail),
+                                        C.byref(f), C.c_void_p(stream))
+        fail = {"code": rc, "cell": f.cell, "ylowa": f.maxerr, "dt": f.dt, "tnow": f.tnow}
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, fail)
+        return fail if rc != 0 else None
+
+    def gear_solve_device(self, chemstep, rki, y, sunlit=None, nsteps=None, nfail=None, stream=None,
+                          raise_on_fail=True):
+        """SMVGEAR with the rate constants given: rki [nr, n], sunlit [n]
+        uint8 (None = all sunlit), y [ns, n] updated in place."""
+        import torch
+        n = y.numel() // self.ns
+        self._check([("rki", rki, self.nr), ("y", y, self.ns), ("nacc", nsteps, 1), ("nrej", nfail, 1)], n)
+        if sunlit is not None and not (sunlit.is_cuda and sunlit.dtype == torch.uint8 and sunlit.numel() == n
+                                       and sunlit.is_contiguous()):
+            raise ValueError("sunlit must be a contiguous uint8 CUDA tensor of n elements")
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_gear_solve_device(self._ctx, n, chemstep, _ptr(rki), _ptr(sunlit), _ptr(y), _ptr(nsteps),
+                                          _ptr(nfail), C.byref(f), C.c_void_p(stream))
+        fail = {"code": rc, "cell": f.cell, "ylowa": f.maxerr, "dt": f.dt, "tnow": f.tnow}
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, fail)
+        return fail if rc != 0 else None
+
+    def gear_last_timing(self):
+        r, s = C.c_double(), C.c_double()
+        lib().cmaq_gear_last_timing(self._ctx, C.byref(r), C.byref(s))
+        return r.value, s.value
 
     def solve_device_noalloc(self, chemstep, rki, y, nacc=None, nrej=None, stream=None, raise_on_fail=True):
         """solve_device through the allocation-free entry point (its

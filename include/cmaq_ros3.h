@@ -450,6 +450,74 @@ int cmaq_ros3_species_order(const cmaq_ros3_ctx *ctx, int32_t *inew2old);
 int cmaq_ros3_get_fail_detail(const cmaq_ros3_ctx *ctx, cmaq_ros3_fail_detail *d, double *spc_err,
                               double *y, double *yp, double *rj, double *rki);
 
+/* ---- SMVGEAR (CCTM/src/gas/smvgear) --------------------------------------
+ * The Gear solver of CMAQ's smvgear CHEM driver on the same context: the
+ * mechanism's tables, species order and sparse pattern are shared with ROS3
+ * (grsprse.F's JSPARSE is RBSPARSE's algorithm).  Drop-in for the block loop
+ * of grdriver.F:400-560 -- CALC_RCONST (grdriver.F:450) + SMVGEAR
+ * (grsmvgear.F:27, grdriver.F:520) -- over every cell in one call, with
+ * per-cell step and order control (NUMCELLS = 1), LSUNLIGHT per cell (any
+ * J > 0) choosing HMAX (15 / 5 min) and the first-step factor H0FAC.
+ * Concentrations enter as MAX(y, CONCMIN) (grdriver.F:419-422).
+ */
+
+/* Gear tolerances, REAL(4) as RUNTIME_VARS.F:173-176 holds them: GEAR_RTOL,
+ * GEAR_MAX_ATOL, GEAR_MIN_ATOL, GEAR_CONC_FLOOR (defaults 1e-3, 1e-7, 1e-9,
+ * 1e-12 ppm); the absolute tolerance of each cell is chosen among six levels
+ * between the two bounds (OPTIMAL_ATOL_PPM, GRVARS.F:550-707). */
+typedef struct cmaq_gear_tol {
+    float rtol, max_atol, min_atol, conc_floor;
+} cmaq_gear_tol;
+
+/* Sets the context's Gear tolerances (NULL = the defaults).  Requires
+ * max_atol > min_atol > conc_floor > 0 and rtol > 0, as OPTIMAL_ATOL_PPM
+ * does (GRVARS.F:588-605, where the reference stops the run). */
+int cmaq_gear_set_tolerances(cmaq_ros3_ctx *ctx, const cmaq_gear_tol *tol);
+
+/* CALC_RCONST + SMVGEAR over ncells cells, host arrays, synchronous; the
+ * arguments as cmaq_ros3_run.  n_steps receives the cell's successful Gear
+ * steps (NSTEPS), n_fail its error-test plus convergence failures (NEFAIL +
+ * NCFAIL).  Where the reference would stop the run (YLOWA lowered
+ * MXLOWRED = 20 times, grsmvgear.F:425-466, or 100 restarts,
+ * grsmvgear.F:768-772) the cell gets n_steps = -1 and the call returns
+ * CMAQ_ROS3_E_CONVERGENCE with *fail holding the cell, DELT (dt), XELAPS
+ * (tnow) and YLOWA (maxerr); every other cell is integrated. */
+int cmaq_gear_run(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min,
+                  const double *temp_k, const double *pres_atm, const double *h2o_ppm,
+                  const double *rj_per_min, const double *het_per_min, const double *seawater,
+                  double *y_ppm, int32_t *n_steps, int32_t *n_fail, cmaq_ros3_fail *fail);
+
+/* Same as cmaq_gear_run on device arrays, enqueued on `stream`; returns
+ * after the stream has finished. */
+int cmaq_gear_run_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min,
+                         const double *temp_k, const double *pres_atm, const double *h2o_ppm,
+                         const double *rj_per_min, const double *het_per_min, const double *seawater,
+                         double *y_ppm, int32_t *n_steps, int32_t *n_fail, cmaq_ros3_fail *fail,
+                         void *stream);
+
+/* SMVGEAR alone with the rate constants supplied (RK of GRVARS, filled by
+ * CALC_RCONST at grdriver.F:450): rki [nrxns][ncells] and sunlit [ncells]
+ * (uint8 LSUNLIGHT per cell; NULL = every cell sunlit), device memory;
+ * otherwise as cmaq_gear_run_device. */
+int cmaq_gear_solve_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min, const double *rki,
+                           const uint8_t *sunlit, double *y_ppm, int32_t *n_steps, int32_t *n_fail,
+                           cmaq_ros3_fail *fail, void *stream);
+
+/* Scheduling of long cells; no counterpart in the reference, whose block
+ * loop runs each block to its end on one core.  A cell still integrating
+ * after after_attempts attempted Gear steps (default 2048; 0 = never) leaves
+ * the cell-per-lane kernel at the top of its next step and is finished by a
+ * second kernel that runs one cell per wavefront with its workspace in LDS,
+ * at a lower latency per step; at most max_cells cells per call do so
+ * (0 = automatic: ncells / 32, at least 1024).  Results do not depend on
+ * either value: both kernels execute the same operations in the same order.
+ * cmaq_ros3_query(ctx, "gear_tail_cells_last") gives the number of cells the
+ * last Gear call handed over. */
+int cmaq_gear_set_tail(cmaq_ros3_ctx *ctx, int32_t after_attempts, int32_t max_cells);
+
+/* Device time (ms) of the last SMVGEAR call's rates and Gear kernels. */
+int cmaq_gear_last_timing(const cmaq_ros3_ctx *ctx, double *rates_ms, double *solve_ms);
+
 /* Message for the last failure on this context (static storage, never NULL). */
 const char *cmaq_ros3_last_error(const cmaq_ros3_ctx *ctx);
 

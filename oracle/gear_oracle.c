@@ -21,6 +21,7 @@
  * here, promoted to double where the Fortran promotes them.
  */
 #include <math.h>
+#include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -45,6 +46,27 @@
 #define PT2 0.2
 #define PT003 0.003
 #define OM_GEAR_MAX_STEPS 10000000L
+
+/* Test knobs of the pow() of the step/order selection (grsmvgear.F:
+ * 880-905).  By default it is libm's pow, as in the reference's build.
+ * om_gear_set_pow_exact(1) takes x^y rounded once from quad precision
+ * (libquadmath's powq, 113-bit), the double nearest the exact value up to
+ * about 2^-60 of an ulp: the result of any correctly rounded pow, which is
+ * what the device's step control computes.  om_gear_set_pow_nudge(n) then
+ * moves every result by n ulps.  tests/test_oracle_gear.py uses both to
+ * measure how far a last-bit difference of pow -- as between two libms --
+ * moves SMVGEAR's results. */
+static int g_pow_nudge = 0;
+static int g_pow_exact = 0;
+void om_gear_set_pow_nudge(int ulps) { g_pow_nudge = ulps; }
+void om_gear_set_pow_exact(int on) { g_pow_exact = on; }
+static double gpow(double a, double b)
+{
+    double r = g_pow_exact ? (double)powq((__float128)a, (__float128)b) : pow(a, b);
+    for (int k = 0; k < g_pow_nudge; ++k) r = nextafter(r, INFINITY);
+    for (int k = 0; k > g_pow_nudge; --k) r = nextafter(r, -INFINITY);
+    return r;
+}
 
 /* Gear coefficient tables, as JSPARSE leaves them (grsprse.F:675-713,
  * GRVARS.F:251-254): PERTST squared, ENQQ1..3 from REAL(4) divisions,
@@ -528,13 +550,13 @@ int om_smvgear(const om_ctx *c, int ncsp, int nc, double chemstep, const double 
                         double der3max = 0.0;
                         for (int cc = 0; cc < nc; ++cc)
                             if (dely[cc] > der3max) der3max = dely[cc];
-                        double pr3 = (double)1.4f * pow(der3max / eup, g.enqq3[nqq]);
+                        double pr3 = (double)1.4f * gpow(der3max / eup, g.enqq3[nqq]);
                         rdeltup = 1.0 / (pr3 + OPT4MIC);
                     } else {
                         rdeltup = 0.0;
                     }
                 }
-                double pr2 = 1.2 * pow(der2max / enqq, g.enqq2[nqq]);
+                double pr2 = 1.2 * gpow(der2max / enqq, g.enqq2[nqq]);
                 double rdeltsm = 1.0 / (pr2 + OPT2MIC);
                 double rdeltdn;
                 if (nqq > 1) {
@@ -549,7 +571,7 @@ int om_smvgear(const om_ctx *c, int ncsp, int nc, double chemstep, const double 
                     double der1max = 0.0;
                     for (int cc = 0; cc < nc; ++cc)
                         if (dely[cc] > der1max) der1max = dely[cc];
-                    double pr1 = 1.3 * pow(der1max / edwn, g.enqq1[nqq]);
+                    double pr1 = 1.3 * gpow(der1max / edwn, g.enqq1[nqq]);
                     rdeltdn = 1.0 / (pr1 + OPT3MIC);
                 } else {
                     rdeltdn = 0.0;

@@ -102,6 +102,10 @@ def lib():
         L.om_ddm_step.argtypes = [vp, C.c_double, _f64p, _f64p, C.c_int, _i32p, C.POINTER(C.c_float)]
         L.om_ddm_step.restype = None
         L.om_tlm_cell.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, C.c_int, _f64p, _f64p]
+        L.om_gear_set_pow_nudge.argtypes = [C.c_int]
+        L.om_gear_set_pow_nudge.restype = None
+        L.om_gear_set_pow_exact.argtypes = [C.c_int]
+        L.om_gear_set_pow_exact.restype = None
         L.om_gear_coefficients.argtypes = [_f64p] * 4
         L.om_gear_coefficients.restype = None
         L.om_subfun.argtypes = [vp, C.c_int, C.c_int, _f64p, _f64p, _f64p]

@@ -62,7 +62,7 @@ om_ctx *om_create(const om_tables *t, char *err, int errlen)
     for (int h = t->mhetero; h >= 1; --h) c->het_idx[t->ihetero[(h - 1) * 2]] = t->ihetero[(h - 1) * 2 + 1];
     for (int r = 1; r <= nr; ++r) {
         int k = t->ktype[r - 1];
-        if (k == 7 || k == 11 || k == 13 || k < -1 || k > 13) {
+        if (k == 11 || k == 13 || k < -1 || k > 13) {
             snprintf(err, errlen, "reaction %d: rate type %d not supported", r, k);
             om_destroy(c); return NULL;
         }
@@ -424,6 +424,10 @@ int om_rates_block(const om_ctx *c, int nc, const double *temp, const double *pr
                 else RK(nxx) = f_ok ? f * RK(irx) : RK(irx);
                 break;
             }
+            case 7:   /* wrt_reactions_module.f:731-737, formats 5007 and 5114: no unit factor */
+                if (b != 0.0) RK(nxx) = arr_t04(inv_temp, t300, d12_4(a), 0.0, d12_4(b)) * pres[cc];
+                else RK(nxx) = d12_4(a) * (1.0 + 0.6 * pres[cc]);
+                break;
             case 8:
                 v = fall_t08(inv_temp, cair, a, b, cc3, RFDAT(c, 1, fi), RFDAT(c, 2, fi), RFDAT(c, 3, fi));
                 RK(nxx) = conv_factor(&u, ord, &f) ? f * v : v;

@@ -174,6 +174,12 @@ typedef struct om_gear_stats {
     int mxordused;
 } om_gear_stats;
 
+/* Test knobs of the pow() of SMVGEAR's step/order selection, for measuring
+ * the reference's sensitivity to libm: `on` = 1 takes x^y rounded from quad
+ * precision (a correctly rounded pow) instead of libm's; then every result
+ * is moved by `ulps` (0 = off). */
+void om_gear_set_pow_nudge(int ulps);
+void om_gear_set_pow_exact(int on);
 /* ASET (6 x 7), squared PERTST (7 x 3), ENQQ1..3 (7 x 3), CONPST / CONP15
  * (7 x 2) as JSPARSE leaves them (grsprse.F:675-713), row-major by order. */
 void om_gear_coefficients(double *aset, double *pertst, double *enqq, double *conpst);

@@ -6,6 +6,7 @@
 !   ros3_fdriver init  TABLES [NSPECIAL]         cmaq_ros3_init: return code, message
 !   ros3_fdriver run   TABLES INPUTS OUT LOG     init + cmaq_ros3_run; outputs to OUT,
 !                                                a failure dump to LOG
+!   ros3_fdriver gear  TABLES INPUTS OUT         init + cmaq_gear_run; outputs to OUT
 program ros3_fdriver
    use, intrinsic :: iso_c_binding
    use cmaq_ros3_c
@@ -56,9 +57,22 @@ program ros3_fdriver
          stop 2
       end if
       call run_cells(fin, fout, flog)
+   case ('gear')
+      call get_command_argument(2, ftab)
+      call get_command_argument(3, fin)
+      call get_command_argument(4, fout)
+      call read_tables(ftab)
+      nspecial = 0
+      call fill_mech()
+      rc = cmaq_ros3_init(ctx, mech, rtol, atol, opts)
+      if (rc /= CMAQ_ROS3_OK) then
+         write (*, '(a, i0, 1x, a)') 'init failed ', rc, cmaq_ros3_error_message(ctx)
+         stop 2
+      end if
+      call gear_cells(fin, fout)
       call cmaq_ros3_free(ctx)
    case default
-      write (*, '(a)') 'usage: ros3_fdriver sizes | init TABLES [NSPECIAL] | run TABLES INPUTS OUT LOG'
+      write (*, '(a)') 'usage: ros3_fdriver sizes | init TABLES [NSPECIAL] | run TABLES INPUTS OUT LOG | gear TABLES INPUTS OUT'
       stop 1
    end select
 
@@ -75,12 +89,14 @@ contains
       type(cmaq_ros3_fail_t), target :: f
       type(cmaq_ros3_fail_detail_t), target :: d
       type(cmaq_ros3_grid_t), target :: g
+      type(cmaq_gear_tol_t), target :: t
       integer(c_intptr_t) :: b
       write (*, '(a, i0)') 'sizeof mech ', c_sizeof(m)
       write (*, '(a, i0)') 'sizeof opts ', c_sizeof(o)
       write (*, '(a, i0)') 'sizeof fail ', c_sizeof(f)
       write (*, '(a, i0)') 'sizeof fail_detail ', c_sizeof(d)
       write (*, '(a, i0)') 'sizeof grid ', c_sizeof(g)
+      write (*, '(a, i0)') 'sizeof gear_tol ', c_sizeof(t)
       b = addr(c_loc(m))
       write (*, '(a, i0)') 'offset mech.nrxns ', addr(c_loc(m%nrxns)) - b
       write (*, '(a, i0)') 'offset mech.irr ', addr(c_loc(m%irr)) - b
@@ -109,6 +125,8 @@ contains
       b = addr(c_loc(g))
       write (*, '(a, i0)') 'offset grid.cgrid ', addr(c_loc(g%cgrid)) - b
       write (*, '(a, i0)') 'offset grid.khetero ', addr(c_loc(g%khetero)) - b
+      b = addr(c_loc(t))
+      write (*, '(a, i0)') 'offset gear_tol.conc_floor ', addr(c_loc(t%conc_floor)) - b
    end subroutine print_sizes
 
    ! Stream file written by the test: sizes, then each table in Fortran order.
@@ -199,5 +217,29 @@ contains
       write (u) rc, y, nacc, nrej
       close (u)
    end subroutine run_cells
+
+   ! The same cells through SMVGEAR with the default Gear tolerances.
+   subroutine gear_cells(pin, pout)
+      character(len=*), intent(in) :: pin, pout
+      integer(c_int64_t) :: n
+      real(c_double) :: chemstep
+      real(c_double), allocatable :: temp(:), pres(:), h2o(:), rj(:, :), y(:, :)
+      integer(c_int32_t), allocatable :: nsteps(:), nfail(:)
+      type(cmaq_ros3_fail_t) :: fail
+      type(cmaq_gear_tol_t) :: tol
+      integer :: u
+      open (newunit=u, file=trim(pin), access='stream', form='unformatted', status='old')
+      read (u) n, chemstep
+      allocate (temp(n), pres(n), h2o(n), rj(n, max(nphotab, 1)), y(n, ns), nsteps(n), nfail(n))
+      read (u) temp, pres, h2o, rj, y
+      close (u)
+      rc = cmaq_gear_set_tolerances(ctx, tol)
+      if (rc == CMAQ_ROS3_OK) &
+         rc = cmaq_gear_run(ctx, n, chemstep, temp, pres, h2o, rj, c_null_ptr, c_null_ptr, y, nsteps, nfail, fail)
+      write (*, '(a, i0, a, i0)') 'rc ', rc, ' cell ', fail%cell
+      open (newunit=u, file=trim(pout), access='stream', form='unformatted', status='replace')
+      write (u) rc, y, nsteps, nfail
+      close (u)
+   end subroutine gear_cells
 
 end program ros3_fdriver

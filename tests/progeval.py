@@ -11,6 +11,8 @@ CONST = dict(GAM=0.43586652150845899941601945119356e+00, C21=-0.1015617108387770
              B1=1.0, B2=0.61697947043828245592553615689730e+01, B3=-0.42772256543218573326238373806514e+00,
              CONMIN=1.0e-30)
 _TERN = re.compile(r"^(.*?) = (.*) > CONMIN \? (.*) : 0\.0;$")
+# SMVGEAR's clamp of GLOSS at species at or below ZBOUND (grsubfun.F:165-171)
+_GCLAMP = re.compile(r"^(.*?) = \((.*) <= ZBOUND && (.*) < 0\.0\) \? 0\.0 : (.*);$")
 
 
 def _name(a):
@@ -22,7 +24,10 @@ def compile_ops(ops, params=()):
     for op in ops:
         s = op.fmt.format(*[_name(a) for a in op.args])
         m = _TERN.match(s)
-        if m:
+        g = _GCLAMP.match(s)
+        if g:
+            s = f"{g.group(1)} = 0.0 if ({g.group(2)} <= ZBOUND and {g.group(3)} < 0.0) else {g.group(4)}"
+        elif m:
             s = f"{m.group(1)} = {m.group(3)} if ({m.group(2)}) > CONMIN else 0.0"
         lines.append("    " + s.rstrip(";"))
     src = f"def _f(W, {', '.join(params) + ', ' if params else ''}**_):\n"
@@ -30,6 +35,8 @@ def compile_ops(ops, params=()):
     src += "".join(f"    {a} = W['{a}']\n" for a in arrays)
     src += "\n".join(lines) + "\n"
     ns = dict(CONST)
+    ns["ZBOUND"] = 1.0e-30
+    ns["opaque"] = lambda v: v
     exec(compile(src, "<ops>", "exec"), ns)
     return ns["_f"]
 

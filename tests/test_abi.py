@@ -16,7 +16,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 
 def _declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(cmaq_ros3_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(cmaq_(?:ros3|gear)_[a-z_]+)\s*\(", text)))
 
 
 def _lib():
@@ -89,15 +89,21 @@ def test_invalid_tables(cb6r5, field, value, expect):
     assert expect.lower() in msg.lower(), msg
 
 
-def test_reserved_rate_types_rejected(cb6r5):
-    """KTYPE 7 / 11 / 13 (SPECIAL and Fortran-text rates) do not occur in the
-    shipped mechanisms and are refused rather than evaluated wrongly."""
+@pytest.mark.parametrize("ktype", [11, 13])
+def test_reserved_rate_types_rejected(cb6r5, ktype):
+    """KTYPE 11 (a SPECIAL operator: a rate recomputed from the
+    concentrations in every RBFEVAL) and 13 (Fortran text that chemmech
+    writes into CALC_RCONST only, absent from RXNS_DATA) are refused rather
+    than evaluated wrongly; KTYPE 7 is accepted."""
     L = _lib()
     m = copy.deepcopy(cb6r5)
     r = m["ktype"].index(3)
-    m["ktype"][r] = 13
+    m["ktype"][r] = ktype
     rc, msg = _init_rc(L, m)
     assert rc == -2 and "type" in msg.lower(), msg
+    m["ktype"][r] = 7
+    rc, msg = _init_rc(L, m)
+    assert rc in (0, -3), msg
 
 
 def test_python_binding_matches_header():
@@ -105,7 +111,7 @@ def test_python_binding_matches_header():
     its prototype in include/cmaq_ros3.h."""
     L = _lib()
     text = open(HEADER).read()
-    decls = re.findall(r"\n(?:int|void|const char \*)\s*(cmaq_ros3_\w+)\s*\(([^;]*?)\);", text, re.S)
+    decls = re.findall(r"\n(?:int|void|const char \*)\s*(cmaq_(?:ros3|gear)_\w+)\s*\(([^;]*?)\);", text, re.S)
     assert len(decls) == len(_declared())
     for name, params in decls:
         n = 0 if params.strip() in ("", "void") else params.count(",") + 1
@@ -119,17 +125,22 @@ def test_library_carries_device_code_of_every_kernel():
     registry record; reverse kernel).  The library's gfx950 code objects must
     hold the kernel descriptor of every kernel the registry record points at:
     the forward, tape-recording, tangent-linear and YAVE-accumulating
-    instantiations of solve_kernel_t and reverse_kernel."""
-    from cmaq_amd.build import VARIANTS, variant_name
+    instantiations of solve_kernel_t and reverse_kernel, and for each
+    mechanism's default variant the SMVGEAR kernels (a third unit)."""
+    from cmaq_amd.build import VARIANTS, tuned, variant_name, variants_of
     _lib()
     blob = open(ros3.LIB_PATH, "rb").read()
     missing = []
-    for name, variants in VARIANTS.items():
-        for (b, l, order) in variants:
+    for name in VARIANTS:
+        for (b, l, order) in variants_of(name):
             c = f"{name}_{variant_name(b, l, order)}"
             ns = f"_ZN4ros3{len(c)}{c}"
             kds = [f"{ns}14solve_kernel_tILb{tape}ELb{tlm}ELb{yave}EEEvNS_11SolveParamsE.kd"
                    for tape, tlm, yave in ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1))]
             kds.append(f"{ns}14reverse_kernelENS_11SolveParamsE.kd")
+            if variant_name(b, l, order) == tuned(name)[0]:
+                g = c + "_gear"
+                kds.append(f"_ZN4ros3{len(g)}{g}11gear_kernelENS_10GearParamsE.kd")
+                kds.append(f"_ZN4ros3{len(g)}{g}4tail16gear_tail_kernelENS_10GearParamsE.kd")
             missing += [k for k in kds if k.encode() not in blob]
     assert not missing, missing

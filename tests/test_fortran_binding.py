@@ -116,10 +116,12 @@ def test_types_match_the_c_header(tmp_path, tmp_path_factory):
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cmaq_ros3.h"', "int main(void) {"]
     for (kind, name) in got:
         if kind == "sizeof":
-            lines.append(f'printf("sizeof {name} %zu\\n", sizeof(cmaq_ros3_{name}));')
+            ctype = "cmaq_gear_tol" if name == "gear_tol" else f"cmaq_ros3_{name}"
+            lines.append(f'printf("sizeof {name} %zu\\n", sizeof({ctype}));')
     for (_, name) in fields:
         t, fld = name.split(".")
-        lines.append(f'printf("offset {name} %zu\\n", offsetof(cmaq_ros3_{t}, {fld}));')
+        ctype = "cmaq_gear_tol" if t == "gear_tol" else f"cmaq_ros3_{t}"
+        lines.append(f'printf("offset {name} %zu\\n", offsetof({ctype}, {fld}));')
     lines.append("return 0; }")
     csrc.write_text("\n".join(lines) + "\n")
     exe = tmp_path / "sizes"
@@ -194,3 +196,24 @@ def test_fortran_driver_run_matches_oracle(tmp_path, tmp_path_factory, cb6r5, or
     assert max(rows)[1] == "PAR" and max(rows)[0] > 0.9
     assert abs(sum(r[0] for r in rows) - 1.0) < 1e-6       # the shares of ERR add up to 1
     assert "in cell       8" in text
+
+
+@pytest.mark.gpu
+def test_fortran_driver_gear_matches_oracle(tmp_path, tmp_path_factory, cb6r5, oracle_cb6r5):
+    """256 cells through the Fortran driver's cmaq_gear_set_tolerances +
+    cmaq_gear_run (SMVGEAR), against the oracle's grsmvgear restatement."""
+    import cmaq_amd.inputs as I
+    _driver(tmp_path_factory)
+    m, o = cb6r5, oracle_cb6r5
+    tab, fin, fout = (tmp_path / x for x in ("t.tab", "in.bin", "out.bin"))
+    write_tables(tab, m)
+    n = 256
+    d = I.make_cells(m, np.arange(n), seed=8, workload="clean")
+    write_inputs(fin, d)
+    print(_run(["gear", str(tab), str(fin), str(fout)]), flush=True)
+    rc, y, nsteps, nfail = read_outputs(fout, m["numb_mech_spc"], n)
+    ref = o.gear_run(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], d["y"])
+    assert rc == 0 and ref["rc"] == 0
+    from test_oracle_gear import gear_bar
+    ok, st = gear_bar(ref["y"], y, ref["nsteps"], nsteps, rates_differ=True)     # the library's own CALC_RCONST
+    assert ok, st

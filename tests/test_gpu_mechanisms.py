@@ -109,3 +109,25 @@ def test_adjoint_matches_oracle(mos):
         assert np.abs(sl - rl).max() <= 1e-8 * np.abs(rl).max(), (name, c)
         assert np.abs(sg - rg).max() <= 1e-8 * np.abs(rg).max(), (name, c)
     print(f"adjoint {name}: cells {n}, step counts equal {ok.mean():.4f}")
+
+
+def test_rate_type7_on_gpu(cb6r5):
+    """chemmech type 7 rates (pressure-dependent) in the GPU's rate program
+    against the oracle's restatement of the generated CALC_RCONST text."""
+    import torch
+    from oracle.oracle import Oracle
+    from cmaq_amd.ros3 import Ros3Solver
+    from test_oracle import _ktype7_mech
+    m, r1, r2 = _ktype7_mech(cb6r5)
+    o, s = Oracle(m), Ros3Solver(m)
+    n = 256
+    d = make_cells(m, np.arange(n), seed=6)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+    rki = torch.zeros((m["nrxns"], n), dtype=torch.float64, device="cuda")
+    s.rates_device(dev(d["temp"]), dev(d["pres"]), dev(d["h2o"]), dev(d["rj"]), rki)
+    torch.cuda.synchronize()
+    g = rki.cpu().numpy()
+    for c in range(0, n, 16):
+        r, _ = o.rates(d["temp"][c:c + 1], d["pres"][c:c + 1], d["h2o"][c:c + 1], d["rj"][:, c:c + 1])
+        np.testing.assert_allclose(g[[r1, r2], c], r[[r1, r2], 0], rtol=1e-14, atol=0)
+        np.testing.assert_allclose(g[:, c], r[:, 0], rtol=1e-14, atol=0)

@@ -338,3 +338,31 @@ def test_stiffness_prepass_matches_feval(cb6r5, oracle_cb6r5):
     # night pattern only drops photolysis terms that are zero at J = 0
     e50 = o.stiffness(d["temp"], d["pres"], d["h2o"], d["rj"], d["y"], rtol, atol, blksize=50)
     assert np.allclose(e50, e1, rtol=1e-12, atol=0)
+
+
+def _ktype7_mech(cb6r5):
+    """CB6r5 with two reactions outside every multiplier list turned into
+    chemmech type 7 rates: A*(1 + 0.6 PRESS) (RTDAT(2) = 0) and
+    ARRHENUIS_T04(INV_TEMP, TEMPOT300, A, 0, B) * PRESS (RTDAT(2) = B),
+    wrt_reactions_module.f:731-737 with formats 5007 and 5114."""
+    import copy
+    m = copy.deepcopy(cb6r5)
+    mult = set()
+    for k in ("nrxwm", "nrxww", "nrxwo2", "nrxwn2", "nrxwch4", "nrxwh2"):
+        mult |= {r - 1 for r in m[k]}
+    cand = [r for r in range(m["nrxns"]) if m["ktype"][r] == 3 and r not in mult]
+    r1, r2 = cand[0], cand[1]
+    m["ktype"][r1], m["rtdat"][r1] = 7, [2.5e-13, 0.0, 0.0]
+    m["ktype"][r2], m["rtdat"][r2] = 7, [1.25e-12, -1.5, 0.0]
+    return m, r1, r2
+
+
+def test_rate_type7_closed_form(cb6r5):
+    from oracle.oracle import Oracle
+    m, r1, r2 = _ktype7_mech(cb6r5)
+    o = Oracle(m)
+    d = make_cells(m, np.arange(16), seed=4)
+    rki, _ = o.rates(d["temp"], d["pres"], d["h2o"], d["rj"])
+    p, t = d["pres"], d["temp"]
+    np.testing.assert_allclose(rki[r1], 2.5e-13 * (1.0 + 0.6 * p), rtol=1e-15, atol=0)
+    np.testing.assert_allclose(rki[r2], 1.25e-12 * (t / 300.0) ** -1.5 * p, rtol=2e-15, atol=0)

@@ -168,3 +168,113 @@ def test_block_of_one_equals_single_cell_calls(cb6r5, oracle_cb6r5):
     r50 = o.gear_run(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], d["y"], blksize=50)
     assert r50["rc"] == 0
     assert len(set(r50["nsteps"][:30].tolist())) == 1      # one step sequence per block
+
+
+def test_night_pattern_equals_day_pattern(cb6r5, oracle_cb6r5):
+    """With J = 0 the day pattern's extra entries carry exact zeros, so a
+    night cell integrated on the day pattern (as the GPU does for every cell)
+    equals the reference's night pattern (NCSP = 2) bit for bit."""
+    o = oracle_cb6r5
+    d, rki, y = _state(cb6r5, o, seed=12, n=16)
+    night = np.flatnonzero(~(d["rj"] > 0).any(axis=0))
+    assert night.size > 0
+    for c in night[:3]:
+        r1 = o.smvgear(1, 5.0, rki[:, c:c + 1], np.maximum(y[:, c:c + 1], 1e-30), False)
+        r2 = o.smvgear(2, 5.0, rki[:, c:c + 1], np.maximum(y[:, c:c + 1], 1e-30), False)
+        assert r1[0] == r2[0] == 0
+        assert np.array_equal(r1[1], r2[1]) and r1[2] == r2[2]
+
+
+def gear_bar(y_ref, y, steps_ref, steps, rtol=1.0e-3, atol=1.0e-9, rates_differ=False):
+    """SMVGEAR parity bar for results that may differ in the last bit of a
+    pow() of the step control or, with rates_differ, of the rate constants.
+
+    Same rate constants: cells whose step count matches are within
+    0.1 * (RTOL |c| + ATOL); the rest -- cells where the controller's discrete
+    choices (order, Jacobian reuse, corrector convergence) went another way
+    after a last-bit difference -- within 100 * (RTOL |c| + ATOL); and the
+    step counts match in >= 99 % of cells (test_reference_sensitivity_to_pow_ulps).
+
+    Rate constants that differ in the last bit (another libm's exp and pow in
+    CALC_RCONST): the step counts match in >= 98 % of cells, no more than 2 %
+    of cells differ by more than 0.1 * (RTOL |c| + ATOL) and none by more than
+    1000 x -- the reference differs from itself by up to ~450 x, also in cells
+    whose step count does not change, when its rate constants move by one ulp
+    (test_reference_sensitivity_to_rate_ulps).  Returns (ok, summary)."""
+    q = np.abs(y - y_ref) / (rtol * np.abs(y_ref) + atol)
+    same = steps == steps_ref
+    close = (q <= 0.1).all(axis=0)
+    if rates_differ:
+        ok = same.mean() >= 0.98 and (~close).mean() <= 0.02 and bool((q <= 1000.0).all())
+    else:
+        ok = same.mean() >= 0.99 and bool(close[same].all()) and bool((q <= 100.0).all())
+    return ok, {"step_match": float(same.mean()), "max_q_matching": float(q[:, same].max(initial=0.0)),
+                "max_q": float(q.max(initial=0.0)), "beyond_0.1": float((~close).mean())}
+
+
+def test_reference_sensitivity_to_rate_ulps(cb6r5, oracle_cb6r5):
+    """The reference against itself when every rate constant moves by at most
+    one ulp (a random direction per constant; the difference between two
+    libms' exp and pow in CALC_RCONST): a percent of cells take another step
+    sequence or another order sequence of the same length, and differ by up
+    to a few hundred (RTOL |c| + ATOL).  The bar for a GPU run with its own
+    rate constants (gear_bar with rates_differ) is set from this."""
+    o = oracle_cb6r5
+    n = 1024
+    d = make_cells(cb6r5, np.arange(n), seed=5, workload="clean")
+    rki = np.zeros((cb6r5["nrxns"], n))
+    for c in range(n):
+        rki[:, c:c + 1], _ = o.rates(d["temp"][c:c + 1], d["pres"][c:c + 1], d["h2o"][c:c + 1], d["rj"][:, c:c + 1])
+    r0 = o.gear_run(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], d["y"], rki=rki)
+    mv = np.random.default_rng(0).integers(-1, 2, size=rki.shape)
+    rk = np.where(mv > 0, np.nextafter(rki, np.inf), np.where(mv < 0, np.nextafter(rki, -np.inf), rki))
+    r1 = o.gear_run(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], d["y"], rki=rk)
+    ok = (r0["nsteps"] >= 0) & (r1["nsteps"] >= 0)
+    good, s = gear_bar(r0["y"][:, ok], r1["y"][:, ok], r0["nsteps"][ok], r1["nsteps"][ok], rates_differ=True)
+    print(s)
+    assert good, s
+    assert s["step_match"] < 1.0 and s["max_q"] > 100.0          # beyond the same-rates bar
+
+
+def test_reference_sensitivity_to_pow_ulps(cb6r5, oracle_cb6r5):
+    """SMVGEAR's discrete control makes the algorithm itself sensitive to the
+    last bit of its pow() calls (the step/order selection, grsmvgear.F:
+    880-905): moving every pow result by one ulp -- the difference between
+    two libms -- changes the step count of a few tenths of a percent of
+    cells, the stiffest ones, and their trace species by up to ~65
+    (RTOL |c| + ATOL); every other cell stays within rounding.  The bar
+    against libm's pow (gear_bar) is set from this.  Between glibc's pow and
+    a correctly rounded one the difference is far smaller: glibc misrounds
+    only a few calls in ten thousand."""
+    from oracle.oracle import lib
+    o = oracle_cb6r5
+    d = make_cells(cb6r5, np.arange(1024), seed=11)
+    r0 = o.gear_run(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], d["y"])
+    worst = 0.0
+    for ulps in (1, -1):
+        try:
+            lib().om_gear_set_pow_nudge(ulps)
+            r1 = o.gear_run(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], d["y"])
+        finally:
+            lib().om_gear_set_pow_nudge(0)
+        same = r0["nsteps"] == r1["nsteps"]
+        assert 0.98 <= same.mean() < 1.0                   # the perturbation matters, in few cells
+        assert np.median(r0["nsteps"][~same]) > 3 * np.median(r0["nsteps"])   # the stiff ones
+        ok, s = gear_bar(r0["y"], r1["y"], r0["nsteps"], r1["nsteps"])
+        print(ulps, s)
+        assert ok, s
+        assert s["max_q_matching"] < 1e-3                  # matching cells: rounding only
+        worst = max(worst, s["max_q"])
+    assert worst > 1.0                                      # beyond the plain 0.1 bar
+    # libm's pow against a correctly rounded one (quad-precision powq, the
+    # device's step control): glibc's pow misrounds a few calls in ten
+    # thousand, which moves a few percent of cells by rounding only
+    try:
+        lib().om_gear_set_pow_exact(1)
+        r2 = o.gear_run(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], d["y"])
+    finally:
+        lib().om_gear_set_pow_exact(0)
+    ok, s = gear_bar(r0["y"], r2["y"], r0["nsteps"], r2["nsteps"])
+    identical = np.all(r0["y"] == r2["y"], axis=0).mean()
+    print("correctly rounded", s, identical)
+    assert ok and s["step_match"] >= 0.995 and 0.9 <= identical < 1.0, (s, identical)
