@@ -12,6 +12,17 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _one_blas_thread():
+    """The CPU tests factor small dense matrices (scipy's Radau, numpy
+    solves): one BLAS thread each, so that threads spinning beside a busy
+    host (a parallel build, pytest-xdist workers) do not slow them down a
+    hundredfold."""
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1):
+        yield
+
+
 @pytest.fixture(scope="session")
 def cb6r5():
     from cmaq_amd.ros3 import load_mechanism
