@@ -126,7 +126,7 @@ def generate(names=None, variants=None, gen=GEN, obj=OBJ, gear=True):
     replaces the built set (the autotuner's candidate library)."""
     os.makedirs(gen, exist_ok=True)
     names = names or list(MECHS)
-    srcs, cnames = [], []
+    srcs, cnames, keep = [], [], set()
     for name in names:
         mech = load_mech(name)
         dflt, refill = tuned(name)
@@ -146,10 +146,11 @@ def generate(names=None, variants=None, gen=GEN, obj=OBJ, gear=True):
                 path = os.path.join(gen, f"ros3_{cname}{suffix}.hip")
                 _write_if_changed(path, t)
                 srcs.append((path, KERNEL_FLAGS))
+                keep.add(f"ros3_{cname}{suffix}")
             cnames.append(cname)
     if set(names) == set(MECHS) or variants:
-        # drop sources and objects of variants no longer in the table
-        keep = {f"ros3_{c}{u}" for c in cnames for u in ("", "_adj", "_gear")}
+        # drop sources and objects of units no longer generated (variants
+        # out of the table, the SMVGEAR unit of a former default)
         for d, ext in ((gen, ".hip"), (obj, ".o")):
             for f in os.listdir(d) if os.path.isdir(d) else []:
                 stem = f[:f.index(".")] if "." in f else f

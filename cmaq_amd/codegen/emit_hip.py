@@ -561,20 +561,16 @@ def emit_gear(m, cname, budget=None, lookahead=None, jac_order=None):
     a("__shared__ double gear_lds[NWS];")
     a("static_assert(NWS * 8 <= 160 * 1024, \"one cell's workspace fits the LDS of a CU\");")
     a("#define GEAR_TAIL")
-    a("#define ROS3_PROLOGUE (void)base_;")
-    # Every access takes the LDS base through an empty asm: the compiler then
-    # keeps the allocator's loads and stores as they are (with constant
-    # addresses it would forward stored values into registers the allocator
-    # had freed, and spill)
+    # Every access takes the LDS base through an empty asm that redefines
+    # it: the compiler then keeps the allocator's loads and stores as they
+    # are (with constant addresses it would forward stored values into the
+    # registers the allocator had freed, and spill), and the base stays in
+    # one VGPR, so an access is one ds_read / ds_write with a constant offset
     a("typedef __attribute__((address_space(3))) double lds_double;")
-    a("__device__ __forceinline__ lds_double *lds_base()")
-    a("{")
-    a("    lds_double *p = (lds_double *)gear_lds;")
-    a("    asm volatile(\"\" : \"+s\"(p));")
-    a("    return p;")
-    a("}")
-    a("#define LD(k) (lds_base()[(k)])")
-    a("#define ST(k, v) (lds_base()[(k)] = (double)(v))")
+    a("#define ROS3_PROLOGUE lds_double *lb_ = (lds_double *)gear_lds; (void)base_;")
+    a("#define LDS_AT(k) (*({ asm volatile(\"\" : \"+v\"(lb_)); lb_ + (k); }))")
+    a("#define LD(k) LDS_AT(k)")
+    a("#define ST(k, v) (LDS_AT(k) = (double)(v))")
     w += body
     a("__device__ __noinline__ void store_gear_cell(const GearParams &P, int64_t cell, double *base_)")
     a("{")
@@ -584,7 +580,7 @@ def emit_gear(m, cname, budget=None, lookahead=None, jac_order=None):
         a(f"    P.y[{sym.inew2old[i]} * n + cell] = LD(GOFF_CN + {i});")
     a("}")
     a("#include \"../ros3_gear_kernel.inc\"")
-    for u in ("LD", "ST", "ROS3_PROLOGUE", "GEAR_TAIL"):
+    for u in ("LD", "ST", "LDS_AT", "ROS3_PROLOGUE", "GEAR_TAIL"):
         a(f"#undef {u}")
     a("}  // namespace tail")
     for u in ("CEFF", "ZBOUND"):
