@@ -487,12 +487,22 @@ def emit_gear(m, cname, budget=None, lookahead=None, jac_order=None):
     ns, nr, nnz = sym.ns, sym.nr, sym.day.nnz
     from .programs import gear_pderiv_lu_ops, gear_subfun_ops
     from .regalloc import allocate as _alloc
+    # rows that the generated functions touch: [entry][64 lanes] per wave, so
+    # that a converged wavefront loads 512 contiguous bytes per row
     off, pos = {}, 0
-    for name, size in (("RK", nr), ("RATE", nr), ("CN", ns), ("GL", ns), ("CO", 7 * ns), ("DTL", ns),
-                       ("CE", ns), ("CH", ns), ("LU", nnz), ("VD", ns), ("B0", ns)):
+    for name, size in (("RK", nr), ("RATE", nr), ("CN", ns), ("GL", ns), ("LU", nnz), ("VD", ns)):
         off[name] = pos
         pos += size
     nws = pos
+    # rows only the state machine touches (CONC's 7 columns, DTLOS, CEST,
+    # CHOLD, BLKCONC), which its lanes reach at different labels: contiguous
+    # per lane after the wave's rows, so that a lane's chunk of 16 rows is one
+    # 128-byte line whichever lanes are active
+    loff, lpos = {}, 0
+    for name, size in (("CO", 7 * ns), ("DTL", ns), ("CE", ns), ("CH", ns), ("B0", ns)):
+        loff[name] = lpos
+        lpos += size
+    lws = (lpos + 15) // 16 * 16
     piv = frozenset(("LU", sym.day.diag[j]) for j in range(ns))
     entry_all = lambda slot: True
     stats, body = {}, []
@@ -516,11 +526,13 @@ def emit_gear(m, cname, budget=None, lookahead=None, jac_order=None):
     a("#include \"../ros3_device.h\"")
     a("#include \"../ros3_gear.h\"")
     a(f"namespace ros3 {{ namespace {gname} {{")
-    a(f"constexpr int NS = {ns}, NR = {nr}, NNZ = {nnz}, NWS = {nws};")
+    a(f"constexpr int NS = {ns}, NR = {nr}, NNZ = {nnz}, NWS = {nws}, LWS = {lws};")
     a(f"constexpr int WAVES_PER_EU = {waves_per_eu(budget)};")
     a("static_assert(kGearCols == 7, \"CONC columns\");")
     for k, v in off.items():
         a(f"constexpr int GOFF_{k} = {v};")
+    for k, v in loff.items():
+        a(f"constexpr int LOFF_{k} = {v};")
     a("#define ROS3_PROLOGUE \\")
     a("    double *base = (double *)((((unsigned long long)(unsigned int)__builtin_amdgcn_readfirstlane( \\")
     a("                                  (int)((unsigned long long)base_ >> 32))) << 32) | \\")
@@ -528,9 +540,13 @@ def emit_gear(m, cname, budget=None, lookahead=None, jac_order=None):
     a("                                  (int)(unsigned long long)base_)); \\")
     a("    const __amdgpu_buffer_rsrc_t rs = \\")
     a("        __builtin_amdgcn_make_buffer_rsrc(base, (short)0, NWS * kWave * 8, 0x00020000); \\")
-    a("    const int vo = __lane_id() * 8;")
+    a("    const int vo = __lane_id() * 8; \\")
+    a("    double *const lw_ = base + NWS * kWave + __lane_id() * LWS; \\")
+    a("    (void)lw_;")
     a("#define LD(k) __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, (k) * (kWave * 8), 0))")
     a("#define ST(k, v) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (double)(v)), rs, vo, (k) * (kWave * 8), 0)")
+    a("#define LDL(k) (lw_[(k)])")
+    a("#define STL(k, v) (lw_[(k)] = (double)(v))")
     a("__device__ __forceinline__ double ceff(double v) { return v <= kGearZbound ? 0.0 : v; }   // grpderiv.F:121-129")
     a("#define CEFF(v) ceff(v)")
     a("#define ZBOUND kGearZbound")
@@ -544,7 +560,7 @@ def emit_gear(m, cname, budget=None, lookahead=None, jac_order=None):
     a("    for (int r = 0; r < NR; ++r) ST(GOFF_RK + r, P.rki[r * n + cell]);")
     for i in range(ns):
         a(f"    {{ const double v = P.y[{sym.inew2old[i]} * n + cell]; const double c = v > kGearConcmin ? v : kGearConcmin;"
-          f" ST(GOFF_CN + {i}, c); ST(GOFF_B0 + {i}, c); }}")
+          f" ST(GOFF_CN + {i}, c); STL(LOFF_B0 + {i}, c); }}")
     a("}")
     a("__device__ __noinline__ void store_gear_cell(const GearParams &P, int64_t cell, double *base_)")
     a("{")
@@ -554,12 +570,12 @@ def emit_gear(m, cname, budget=None, lookahead=None, jac_order=None):
         a(f"    P.y[{sym.inew2old[i]} * n + cell] = LD(GOFF_CN + {i});")
     a("}")
     a("#include \"../ros3_gear_kernel.inc\"")
-    for u in ("LD", "ST", "ROS3_PROLOGUE"):
+    for u in ("LD", "ST", "LDL", "STL", "ROS3_PROLOGUE"):
         a(f"#undef {u}")
     # the same functions over one cell's workspace in LDS, for gear_tail_kernel
     a("namespace tail {")
-    a("__shared__ double gear_lds[NWS];")
-    a("static_assert(NWS * 8 <= 160 * 1024, \"one cell's workspace fits the LDS of a CU\");")
+    a("__shared__ double gear_lds[NWS + LWS];")
+    a("static_assert((NWS + LWS) * 8 <= 160 * 1024, \"one cell's workspace fits the LDS of a CU\");")
     a("#define GEAR_TAIL")
     # Every access takes the LDS base through an empty asm that redefines
     # it: the compiler then keeps the allocator's loads and stores as they
@@ -571,6 +587,8 @@ def emit_gear(m, cname, budget=None, lookahead=None, jac_order=None):
     a("#define LDS_AT(k) (*({ asm volatile(\"\" : \"+v\"(lb_)); lb_ + (k); }))")
     a("#define LD(k) LDS_AT(k)")
     a("#define ST(k, v) (LDS_AT(k) = (double)(v))")
+    a("#define LDL(k) LDS_AT(NWS + (k))")
+    a("#define STL(k, v) (LDS_AT(NWS + (k)) = (double)(v))")
     w += body
     a("__device__ __noinline__ void store_gear_cell(const GearParams &P, int64_t cell, double *base_)")
     a("{")
@@ -580,10 +598,10 @@ def emit_gear(m, cname, budget=None, lookahead=None, jac_order=None):
         a(f"    P.y[{sym.inew2old[i]} * n + cell] = LD(GOFF_CN + {i});")
     a("}")
     a("#include \"../ros3_gear_kernel.inc\"")
-    for u in ("LD", "ST", "LDS_AT", "ROS3_PROLOGUE", "GEAR_TAIL"):
+    for u in ("LD", "ST", "LDL", "STL", "LDS_AT", "ROS3_PROLOGUE", "GEAR_TAIL"):
         a(f"#undef {u}")
     a("}  // namespace tail")
     for u in ("CEFF", "ZBOUND"):
         a(f"#undef {u}")
     a(f"}} }} // namespace ros3::{gname}")
-    return "\n".join(w) + "\n", nws
+    return "\n".join(w) + "\n", nws + lws
