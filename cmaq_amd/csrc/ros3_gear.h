@@ -30,6 +30,8 @@ constexpr double kGearOmic = 0.000001;
 constexpr double kGearOpt2mic = 0.0000012;
 constexpr double kGearOpt3mic = 0.0000013;
 constexpr double kGearOpt4mic = 0.0000014;
+// passes a lane's PDERIV + DECOMP may wait for more lanes of its wavefront
+constexpr int kGearPdWait = 2;
 
 // ASET, PERTST (squared), ENQQ1..3, CONPST, CONP15 as JSPARSE leaves them
 // (grsprse.F:675-713; PERTST's DATA statement GRVARS.F:251-254).  ENQQ1..3
