@@ -47,3 +47,30 @@ def test_gear_programs_bit_exact(cb6r5, oracle_cb6r5, order):
         W["GL"] = list(map(float, rhs))
         back(W)
         assert np.array_equal(np.array(W["GL"]), o.solve(1, cc2, vd, rhs[:, None])[:, 0])
+
+
+def test_handoff_record_holds_every_state_variable():
+    """A cell moved from gear_kernel to gear_tail_kernel carries its control
+    state in GearLaneState, filled and read through the GEAR_LANE_* X-macro
+    lists (cmaq_amd/csrc/ros3_gear.h).  Every variable gear_kernel declares
+    for a lane's state machine is in those lists, apart from the queue's
+    bookkeeping (which cell, whether the lane is idle, the queue drained,
+    the hand-off refused) and the wave's PDERIV wait counter."""
+    import os
+    import re
+    csrc = os.path.join(os.path.dirname(os.path.abspath(__file__)), os.pardir, "cmaq_amd", "csrc")
+    header = open(os.path.join(csrc, "ros3_gear.h")).read()
+    listed = set()
+    for lst in ("GEAR_LANE_INTS", "GEAR_LANE_BOOLS", "GEAR_LANE_DBLS"):
+        m = re.search(r"#define " + lst + r"\(X\)((?:[^\n]*\\\n)*[^\n]*)", header)
+        listed |= set(re.findall(r"X\((\w+)\)", m.group(1)))
+    kernel = open(os.path.join(csrc, "ros3_gear_kernel.inc")).read()
+    body = kernel[kernel.index("void gear_kernel(GearParams P)"):]
+    body = body[body.index("const int64_t n = P.ncells;"):body.index("for (;;)")]
+    declared = set()
+    for decl in re.findall(r"^\s*(?:bool|int|int64_t|double)\s+([^;]+);", body, re.M):
+        for part in decl.split(","):
+            declared.add(part.split("=")[0].strip())
+    bookkeeping = {"cell", "need", "drained", "stay", "pd_age"}
+    assert declared - bookkeeping <= listed, sorted(declared - bookkeeping - listed)
+    assert len(listed) >= 37 and listed <= declared, sorted(listed - declared)

@@ -160,6 +160,26 @@ def test_gear_long_cell_handoff(cb6r5, oracle_cb6r5, after, cap):
         s.set_gear_tail(-1, 0)
 
 
+def test_gear_long_cell_handoff_largest_mechanism():
+    """SAPRC07TIC, the largest mechanism (230 species, 929 reactions): its
+    cell's workspace takes 78 KB of LDS in gear_tail_kernel, beyond the 64 KB
+    an LDS instruction's offset reaches.  Every cell handed over after one
+    step; bit-identical to the oracle with a correctly rounded pow."""
+    from cmaq_amd.ros3 import Ros3Solver, load_mechanism
+    from oracle.oracle import Oracle
+    m = load_mechanism("saprc07tic_ae7i_aq")
+    o = Oracle(m)
+    n = 96
+    d = make_cells(m, np.arange(n), seed=4, workload="clean")
+    s = Ros3Solver("saprc07tic_ae7i_aq")
+    s.set_gear_tail(1, 0)
+    yh, nsh, nfh = s.gear_run(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], d["y"], raise_on_fail=False)
+    assert s.query("gear_tail_cells_last") >= n - 2
+    with exact_pow():
+        ref = _oracle_gear(o, d, rki=_gpu_rates(s, m, d))
+    assert_bit_identical(ref, yh, nsh, nfh)
+
+
 def test_gear_ragged_and_empty(cb6r5, oracle_cb6r5, solver_cb6r5):
     import torch
     for n in (1, 63, 65, 130):
