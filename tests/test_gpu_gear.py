@@ -180,6 +180,40 @@ def test_gear_long_cell_handoff_largest_mechanism():
     assert_bit_identical(ref, yh, nsh, nfh)
 
 
+def test_gear_regional_262k_sample(cb6r5, oracle_cb6r5, solver_cb6r5):
+    """262,144 regional cells in one call, as the bench runs them: a few
+    thousand cells pass 2048 attempted steps and finish in gear_tail_kernel.
+    A sample -- every 1024th cell, plus the 48 handed-over cells with the
+    fewest attempts, which keeps the oracle's share to seconds -- is
+    bit-identical to the oracle (correctly rounded pow, the GPU's rate
+    constants); every other cell ends finite and at or above CONCMIN, or
+    fails as the reference would (n_steps = -1)."""
+    import torch
+    from cmaq_amd.inputs import make_cells_torch
+    n = 262144
+    d = make_cells_torch(cb6r5, n, seed=7, workload="regional", nlay_cells=n // 35, device="cuda")
+    y = d["y"].clone()
+    ns_ = torch.zeros(n, dtype=torch.int32, device="cuda")
+    nf = torch.zeros(n, dtype=torch.int32, device="cuda")
+    solver_cb6r5.gear_run_device(5.0, d["temp"], d["pres"], d["h2o"], d["rj"], y, ns_, nf, raise_on_fail=False)
+    torch.cuda.synchronize()
+    handed = solver_cb6r5.query("gear_tail_cells_last")
+    ok = ns_ >= 0
+    assert handed > 0 and float(ok.float().mean()) > 0.99
+    assert bool(torch.isfinite(y[:, ok]).all()) and bool((y[:, ok] >= 1e-30).all())
+    att = (ns_.clamp(min=0) + nf).to(torch.int64)
+    over = torch.nonzero(att > 2048).flatten()
+    over = over[torch.argsort(att[over])[:48]]
+    cells = torch.unique(torch.cat([torch.arange(0, n, 1024, device="cuda"), over]))
+    host = {k: d[k][..., cells].cpu().numpy() for k in ("temp", "pres", "h2o", "rj", "y")}
+    print(f"gear 262144 regional cells: {handed} handed over; sample {cells.numel()} cells, "
+          f"{over.numel()} of them handed over ({int(att[over].min())}-{int(att[over].max())} attempts)")
+    assert over.numel() == 48
+    with exact_pow():
+        ref = _oracle_gear(oracle_cb6r5, host, rki=_gpu_rates(solver_cb6r5, cb6r5, host))
+    assert_bit_identical(ref, y[:, cells].cpu().numpy(), ns_[cells].cpu().numpy(), nf[cells].cpu().numpy())
+
+
 def test_gear_ragged_and_empty(cb6r5, oracle_cb6r5, solver_cb6r5):
     import torch
     for n in (1, 63, 65, 130):
