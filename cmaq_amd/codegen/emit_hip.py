@@ -468,7 +468,8 @@ def emit(m, cname, budget=None, lookahead=None, variant="", is_default=True, ref
     w(f"    ros3::{cname}::TAPE_RS, (const void *)&ros3::{cname}::solve_kernel_t<false, true>,")
     w(f"    ros3::{cname}::OFF_LAM,   // rows of the workspace the forward kernel touches")
     w(f"    (const void *)&ros3::{cname}::solve_kernel_t<false, false, true>,")
-    w(f"    ROS3_GEAR_KERNEL, {gear_nws}, ROS3_GEAR_TAIL_KERNEL}};")
+    w(f"    ROS3_GEAR_KERNEL, {gear_nws}, ROS3_GEAR_TAIL_KERNEL,")
+    w(f"    (const void *)&ros3::{cname}::solve_kernel_t<false, false, false, true>}};")
     w("#undef ROS3_REVERSE_KERNEL")
     w("#undef ROS3_GEAR_KERNEL")
     w("#undef ROS3_GEAR_TAIL_KERNEL")

@@ -68,6 +68,7 @@ struct cmaq_ros3_ctx {
     int grid_replay = 0, grid_reverse = 0;   // adjoint kernels' workgroups
     int grid_tlm = 0;                          // tangent-linear kernel's workgroups
     int grid_yave = 0;                         // YAVE-accumulating forward's workgroups
+    int grid_block = 0;                        // block-semantics forward's workgroups
     const MechKernels *gk = nullptr;           // the record carrying the SMVGEAR kernel
     int grid_gear = 0;                         // SMVGEAR kernel's workgroups
     int grid_gear_tail = 0;                    // gear_tail_kernel's one-wavefront workgroups
@@ -422,7 +423,9 @@ static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
     if (int rc = grid_for(c, mk->replay_kernel, &grid_a)) return rc;
     if (int rc = grid_for(c, mk->reverse_kernel, &grid_b)) return rc;
     if (int rc = grid_for(c, mk->tlm_kernel, &grid_t)) return rc;
-    int grid_g = 0, grid_gt = 0;
+    int grid_g = 0, grid_gt = 0, grid_k = 0;
+    if (mk->block_kernel)
+        if (int rc = grid_for(c, mk->block_kernel, &grid_k)) return rc;
     if (c->gk && c->gk->gear_kernel)
         if (int rc = grid_for(c, c->gk->gear_kernel, &grid_g)) return rc;
     if (c->gk && c->gk->gear_tail_kernel) {
@@ -432,7 +435,7 @@ static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
     }
     const int wpb = kSolveBlock / kWave;
     const int64_t waves =
-        (int64_t)std::max(std::max(std::max(grid, grid_t), std::max(grid_a, grid_b)), std::max(grid_y, grid_g)) * wpb;
+        (int64_t)std::max(std::max(std::max(grid, grid_t), std::max(grid_a, grid_b)), std::max(std::max(grid_y, grid_g), grid_k)) * wpb;
     // per-wave workspace stride: NWS rows of 64 doubles plus an optional pad
     // of rows that staggers the waves' bases across memory channels
     int64_t pad = 0;
@@ -454,6 +457,7 @@ static int configure(cmaq_ros3_ctx *c, const MechKernels *mk)
     c->grid_reverse = grid_b;
     c->grid_tlm = grid_t;
     c->grid_yave = grid_y;
+    c->grid_block = grid_k;
     c->grid_gear = grid_g;
     c->grid_gear_tail = grid_gt;
     c->ws_stride = stride;
@@ -728,9 +732,11 @@ extern "C" int cmaq_ros3_get_fail_detail(const cmaq_ros3_ctx *c, cmaq_ros3_fail_
 
 static int launch_rates(cmaq_ros3_ctx *c, int64_t n, const double *temp, const double *pres,
                         const double *h2o, const double *rj, const double *het, const double *sea,
-                        double *rki, hipStream_t s, uint8_t *sunlit = nullptr)
+                        double *rki, hipStream_t s, uint8_t *sunlit = nullptr, int64_t blk_iavg = 0,
+                        int64_t blk_nblks = 0)
 {
     RatesParams P;
+    P.blk_iavg = blk_iavg; P.blk_nblks = blk_nblks;
     P.ncells = n; P.nr = c->nr; P.nphotab = c->nphotab; P.nhetero = c->nhetero;
     P.nmul = (int32_t)c->mul.size();
     P.ops = c->d_ops; P.mul = c->d_mul;
@@ -921,6 +927,63 @@ extern "C" int cmaq_ros3_run_device(cmaq_ros3_ctx *c, int64_t n, double chemstep
     int rc = launch_rates(c, n, temp, pres, h2o, rj, het, sea, c->d_rki, s);
     if (rc) return rc;
     rc = launch_solve(c, n, chemstep, c->d_rki, y, nacc, nrej, fail, s);
+    note_fail_inputs(c, n, temp, pres, h2o, rj, sea, c->d_rki);
+    return rc;
+}
+
+// The stock block semantics (BLKSIZE blocks, rbinit.F:286-303; the block's
+// LSUNLIGHT, rbdriver.F:497-519; the block's MAXERR, rbsolver.F:443-456),
+// one block per wave.
+extern "C" int cmaq_ros3_run_blocks_device(cmaq_ros3_ctx *c, int64_t n, int32_t blksize, double chemstep,
+                                           const double *temp, const double *pres, const double *h2o,
+                                           const double *rj, const double *het, const double *sea, double *y,
+                                           int32_t *nacc, int32_t *nrej, cmaq_ros3_fail *fail, void *stream)
+{
+    if (int rc = check_run_args(c, n, chemstep, fail)) return rc;
+    if (blksize < 1 || blksize > kWave)
+        return set_err(c, CMAQ_ROS3_E_ARG, "blksize must be 1.." + std::to_string(kWave));
+    if (n > 0 && (!temp || !pres || !h2o || !y || (c->nphotab > 0 && !rj)))
+        return set_err(c, CMAQ_ROS3_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    if (!c->mk->block_kernel || c->grid_block <= 0)
+        return set_err(c, CMAQ_ROS3_E_ARG, "the kernel variant has no block-semantics kernel");
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (c->rki_cells < n) {
+        hipFree(c->d_rki);
+        c->d_rki = nullptr;
+        c->rki_cells = 0;
+        if (hipMalloc((void **)&c->d_rki, (size_t)n * c->nr * sizeof(double)) != hipSuccess)
+            return set_err(c, CMAQ_ROS3_E_NOMEM, "hipMalloc of rate constants failed");
+        c->rki_cells = n;
+    }
+    // NBLKS near-equal blocks of at most blksize cells, the last one holding
+    // the rest (1 .. iavg cells)
+    const int64_t nblks = 1 + (n - 1) / blksize;
+    const int64_t iavg = std::min<int64_t>(1 + (n - 1) / nblks, blksize);
+    HIP_TRY(c, hipMemsetAsync(c->d_small, 0, 64, s));
+    HIP_TRY(c, hipEventRecord(c->ev[0], s));
+    int rc = launch_rates(c, n, temp, pres, h2o, rj, het, sea, c->d_rki, s, nullptr, iavg, nblks);
+    if (rc) return rc;
+    HIP_TRY(c, hipEventRecord(c->ev[1], s));
+    SolveParams P;
+    solve_params(c, P, n, chemstep, c->d_rki, y, nacc, nrej);
+    P.fail_detail = nullptr;   // a block's failure names its first cell, not one cell's state
+    P.blk_iavg = iavg;
+    P.blk_nblks = nblks;
+    void *args[] = {&P};
+    HIP_TRY(c, hipLaunchKernel(c->mk->block_kernel, dim3(c->grid_block), dim3(kSolveBlock), args, 0, s));
+    HIP_TRY(c, hipEventRecord(c->ev[2], s));
+    unsigned char small[64];
+    HIP_TRY(c, hipMemcpyAsync(small, c->d_small, 64, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    float ms_r = 0.f, ms_s = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms_r, c->ev[0], c->ev[1]));
+    HIP_TRY(c, hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
+    c->rates_ms = ms_r;
+    c->solve_ms = ms_s;
+    rc = check_fail(c, small, fail);
+    if (rc) std::fill(c->fd_raw.begin(), c->fd_raw.end(), (double)NAN);
     note_fail_inputs(c, n, temp, pres, h2o, rj, sea, c->d_rki);
     return rc;
 }

@@ -192,6 +192,10 @@ struct RatesParams {
     const double *temp, *pres, *h2o, *rj, *het, *sea;
     double *rki;      // [nr][ncells]
     uint8_t *sunlit;  // [ncells] or nullptr
+    // blocks of cells sharing LSUNLIGHT (rbdriver.F:497-519): blk_nblks
+    // blocks of blk_iavg consecutive cells, the last one holding the rest;
+    // blk_iavg = 0 decides it per cell
+    int64_t blk_iavg, blk_nblks;
 };
 
 // CALC_RCONST for every cell; the rate program is staged in LDS once per
@@ -212,7 +216,15 @@ __global__ __launch_bounds__(256) void rates_kernel(RatesParams P)
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
          c += (int64_t)gridDim.x * blockDim.x) {
         bool sunlit = false;
-        for (int k = 0; k < P.nphotab; ++k) sunlit |= P.rj[k * n + c] > 0.0;
+        if (P.blk_iavg > 0) {   // any J > 0 in the cell's block
+            int64_t b = c / P.blk_iavg;
+            if (b > P.blk_nblks - 1) b = P.blk_nblks - 1;
+            const int64_t c0 = b * P.blk_iavg, c1 = b == P.blk_nblks - 1 ? n : c0 + P.blk_iavg;
+            for (int k = 0; k < P.nphotab; ++k)
+                for (int64_t q = c0; q < c1; ++q) sunlit |= P.rj[k * n + q] > 0.0;
+        } else {
+            for (int k = 0; k < P.nphotab; ++k) sunlit |= P.rj[k * n + c] > 0.0;
+        }
         const double temp = P.temp[c], pres = P.pres[c];
         const double sea = P.sea ? P.sea[c] : 0.0;
         CellConv u;
@@ -275,6 +287,9 @@ struct SolveParams {
     unsigned long long *tape_count;   // records taken from the arena so far
     int64_t tape_cap;         // arena capacity in records
     double *yave;             // YAVE run: [ns][stride] time-averaged state, mechanism order
+    // BLOCK run: blk_nblks blocks of blk_iavg consecutive cells (the last
+    // one holds the rest, at most kWave cells each) sharing one step size
+    int64_t blk_iavg, blk_nblks;
 };
 
 __device__ inline int lane_id() { return __lane_id(); }

@@ -25,6 +25,7 @@ struct MechKernels {
     const void *gear_kernel;          // SMVGEAR kernel (default variants), or nullptr
     int gear_nws;                     // workspace rows (entries) the SMVGEAR kernel uses
     const void *gear_tail_kernel;     // SMVGEAR for long cells, one per wavefront in LDS, or nullptr
+    const void *block_kernel;         // solve_kernel with a step size shared by each block of cells
 };
 
 }  // namespace ros3

@@ -100,6 +100,8 @@ def lib():
         L.cmaq_ros3_init.argtypes = [C.POINTER(vp), C.POINTER(_Mech), _f64p, _f64p, C.POINTER(_Opts)]
         L.cmaq_ros3_run.argtypes = [vp, C.c_int64, C.c_double] + [_f64p] * 7 + [_i32p, _i32p, C.POINTER(_Fail)]
         L.cmaq_ros3_run_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 9 + [C.POINTER(_Fail), vp]
+        L.cmaq_ros3_run_blocks_device.argtypes = ([vp, C.c_int64, C.c_int32, C.c_double] + [vp] * 9 +
+                                                  [C.POINTER(_Fail), vp])
         L.cmaq_ros3_rates_device.argtypes = [vp, C.c_int64] + [vp] * 7 + [vp]
         L.cmaq_ros3_solve_device.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 4 + [C.POINTER(_Fail), vp]
         L.cmaq_ros3_solve_device_noalloc.argtypes = [vp, C.c_int64, C.c_double] + [vp] * 4 + [C.POINTER(_Fail), vp]
@@ -329,6 +331,29 @@ class Ros3Solver:
         rc = lib().cmaq_ros3_run_device(self._ctx, n, chemstep, _ptr(temp), _ptr(pres_atm), _ptr(h2o_ppm),
                                         _ptr(rj), _ptr(het), _ptr(seawater), _ptr(y), _ptr(nacc), _ptr(nrej),
                                         C.byref(f), C.c_void_p(stream))
+        fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
+        if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
+            raise self._err(rc, fail)
+        return fail if rc != 0 else None
+
+    def run_blocks_device(self, chemstep, temp, pres_atm, h2o_ppm, rj, y, blksize=50, nacc=None, nrej=None,
+                          het=None, seawater=None, stream=None, raise_on_fail=True):
+        """run_device with the reference's stock block semantics: blocks of
+        consecutive cells (at most blksize, rbinit.F:286-303) sharing
+        LSUNLIGHT and one step size (rbsolver.F:443-456)."""
+        import torch
+        n = temp.numel()
+        for t in (temp, pres_atm, h2o_ppm, rj, y):
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+                raise ValueError("inputs must be contiguous float64 CUDA tensors")
+        if y.numel() != self.ns * n or rj.numel() != self.nphotab * n:
+            raise ValueError("shape mismatch")
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        f = _Fail()
+        rc = lib().cmaq_ros3_run_blocks_device(self._ctx, n, int(blksize), chemstep, _ptr(temp), _ptr(pres_atm),
+                                               _ptr(h2o_ppm), _ptr(rj), _ptr(het), _ptr(seawater), _ptr(y),
+                                               _ptr(nacc), _ptr(nrej), C.byref(f), C.c_void_p(stream))
         fail = {"code": rc, "cell": f.cell, "maxerr": f.maxerr, "dt": f.dt, "tnow": f.tnow}
         if rc != 0 and (raise_on_fail or rc not in (-4, -5)):
             raise self._err(rc, fail)

@@ -158,6 +158,28 @@ int cmaq_ros3_run_device(cmaq_ros3_ctx *ctx, int64_t ncells, double chemstep_min
                          cmaq_ros3_fail *fail, void *stream);
 
 /*
+ * cmaq_ros3_run_device with the reference's stock block semantics: the
+ * ncells cells, in array order, form NBLKS = 1 + (ncells-1)/blksize blocks of
+ * IAVG = 1 + (ncells-1)/NBLKS consecutive cells, the last block holding the
+ * rest (rbinit.F:286-303, blksize = BLKSIZE of rbdata_mod.F, 50 in the
+ * reference; 1..64 here).  Each block decides LSUNLIGHT as any J > 0 among
+ * its cells (rbdriver.F:497-519) and integrates with one step size, from
+ * MAXERR = the largest error over its cells (rbsolver.F:443-456), so its
+ * cells share their accepted and rejected step counts.  A block that fails
+ * marks all its cells with n_accepted = -1 and *fail names its first cell;
+ * cmaq_ros3_get_fail_detail then returns the cell's inputs and NaN for the
+ * per-cell state.  A caller that wants the LORDERING order (rbdriver.F:
+ * 683-724) permutes the cell arrays before the call.
+ */
+int cmaq_ros3_run_blocks_device(cmaq_ros3_ctx *ctx, int64_t ncells, int32_t blksize,
+                                double chemstep_min, const double *temp_k,
+                                const double *pres_atm, const double *h2o_ppm,
+                                const double *rj_per_min, const double *het_per_min,
+                                const double *seawater, double *y_ppm,
+                                int32_t *n_accepted, int32_t *n_rejected,
+                                cmaq_ros3_fail *fail, void *stream);
+
+/*
  * nsteps consecutive chemistry sync steps (CHEM calls, rbdriver.F:20) with
  * the same met inputs, on device arrays.  Each sync step -- reset of the cell
  * queue, CALC_RCONST, RBSOLVER over all cells, checkpoint -- is captured once

@@ -135,8 +135,9 @@ def test_library_carries_device_code_of_every_kernel():
         for (b, l, order) in variants_of(name):
             c = f"{name}_{variant_name(b, l, order)}"
             ns = f"_ZN4ros3{len(c)}{c}"
-            kds = [f"{ns}14solve_kernel_tILb{tape}ELb{tlm}ELb{yave}EEEvNS_11SolveParamsE.kd"
-                   for tape, tlm, yave in ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1))]
+            kds = [f"{ns}14solve_kernel_tILb{tape}ELb{tlm}ELb{yave}ELb{blk}EEEvNS_11SolveParamsE.kd"
+                   for tape, tlm, yave, blk in ((0, 0, 0, 0), (1, 0, 0, 0), (0, 1, 0, 0), (0, 0, 1, 0),
+                                                (0, 0, 0, 1))]
             kds.append(f"{ns}14reverse_kernelENS_11SolveParamsE.kd")
             if variant_name(b, l, order) == tuned(name)[0]:
                 g = c + "_gear"
